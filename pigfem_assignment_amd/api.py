@@ -1,0 +1,283 @@
+"""Python handles over the C-ABI: HostProblem (pgfh_*) and DeviceContext (pgf_*).
+
+These mirror the reference's driver flow (Tests/Main.cpp): build a mesh with
+inclusions and materials, set Dirichlet conditions, Problem::init, then solve.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class HostProblem:
+    """Mesh + IGFEM preprocessing + dof numbering (pgfh_problem_create)."""
+
+    def __init__(self, kind, N, mins, maxs, inclusions=(), materials=((3.2e9, 0.35),), incl_mat=None,
+                 plane_strain=True, cohesive=False, coh_laws=(), incl_coh_law=None, default_coh_law=0,
+                 bcs=(), tet_weight_scale=1.0, sphere_zlim_as_reference=False):
+        lib = L.lib()
+        self._keep = []
+        cfg = L.HostConfig()
+        cfg.elem_kind = L.PGF_TET4 if kind == "tet4" else L.PGF_QUAD4
+        for d in range(3):
+            cfg.N[d] = int(N[d]) if d < len(N) else 0
+            cfg.mins[d] = float(mins[d]) if d < len(mins) else 0.0
+            cfg.maxs[d] = float(maxs[d]) if d < len(maxs) else 0.0
+        inc = np.ascontiguousarray(np.asarray(inclusions, dtype=np.float64).reshape(-1, 5))
+        cfg.n_incl = inc.shape[0]
+        cfg.incl = L.ptr(inc, C.c_double)
+        im = np.ascontiguousarray(np.full(inc.shape[0], 1, np.int32) if incl_mat is None
+                                  else np.asarray(incl_mat, np.int32))
+        cfg.incl_mat = L.ptr(im, C.c_int32)
+        icl = None if incl_coh_law is None else np.ascontiguousarray(np.asarray(incl_coh_law, np.int32))
+        cfg.incl_coh_law = L.ptr(icl, C.c_int32)
+        mats = np.asarray(materials, dtype=np.float64).reshape(-1, 2)
+        E = np.ascontiguousarray(mats[:, 0])
+        nu = np.ascontiguousarray(mats[:, 1])
+        cfg.n_mat = len(E)
+        cfg.mat_E = L.ptr(E, C.c_double)
+        cfg.mat_nu = L.ptr(nu, C.c_double)
+        cfg.plane_strain = int(bool(plane_strain))
+        cfg.cohesive = int(bool(cohesive))
+        laws = list(coh_laws)
+        ct = np.ascontiguousarray(np.array([lw[0] for lw in laws], np.int32)) if laws else None
+        cp = np.ascontiguousarray(np.array([list(lw[1:]) + [0.0] * (7 - len(lw)) for lw in laws],
+                                           np.float64)[:, :6]) if laws else None
+        cfg.n_coh_law = len(laws)
+        cfg.coh_type = L.ptr(ct, C.c_int32)
+        cfg.coh_params = L.ptr(cp, C.c_double)
+        cfg.default_coh_law = default_coh_law
+        cfg.sphere_zlim_as_reference = int(bool(sphere_zlim_as_reference))
+        cfg.tet_weight_scale = tet_weight_scale
+        bset = np.ascontiguousarray(np.array([b[0] for b in bcs], np.int32))
+        bnode = np.ascontiguousarray(np.array([b[1] for b in bcs], np.int32))
+        bdof = np.ascontiguousarray(np.array([b[2] for b in bcs], np.int32))
+        bval = np.ascontiguousarray(np.array([b[3] for b in bcs], np.float64))
+        cfg.n_bc = len(bcs)
+        cfg.bc_set, cfg.bc_node = L.ptr(bset, C.c_int32), L.ptr(bnode, C.c_int32)
+        cfg.bc_dof, cfg.bc_val = L.ptr(bdof, C.c_int32), L.ptr(bval, C.c_double)
+        self._keep += [inc, im, icl, E, nu, ct, cp, bset, bnode, bdof, bval]
+        h = C.c_void_p()
+        L.check(lib.pgfh_problem_create(C.byref(cfg), C.byref(h)), host=True)
+        self.h = h
+        self.info = L.HostInfo()
+        L.check(lib.pgfh_problem_info(self.h, C.byref(self.info)), host=True)
+        self.materials = mats
+        self.coh_laws = laws
+        self.plane_strain = plane_strain
+        self.kind = kind
+        self.tet_weight_scale = tet_weight_scale
+        self.inclusions = inc
+
+    def bc_list(self):
+        """Dirichlet conditions as (node, dof, full-load value) over regular nodes."""
+        dof, pres = self.dofs()
+        nf = self.info.n_free
+        out = []
+        for d in range(self.dim):
+            for n in np.nonzero(dof[d, : self.info.n_nodes] >= nf)[0]:
+                out.append((int(n), d, float(pres[dof[d, n] - nf])))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            L.lib().pgfh_problem_destroy(self.h)
+            self.h = None
+
+    # ---- exports ----
+    @property
+    def dim(self):
+        return self.info.dim
+
+    def coords(self):
+        a = np.zeros((3, self.info.n_all_nodes))
+        L.lib().pgfh_get_coords(self.h, L.ptr(a, C.c_double))
+        return a
+
+    def conn(self):
+        a = np.zeros((self.info.n_elem, 4), np.int32)
+        L.lib().pgfh_get_conn(self.h, L.ptr(a, C.c_int32))
+        return a
+
+    def elem_mat(self):
+        a = np.zeros(self.info.n_elem, np.int32)
+        L.lib().pgfh_get_elem_mat(self.h, L.ptr(a, C.c_int32))
+        return a
+
+    def node_detect(self):
+        a = np.zeros(self.info.n_nodes, np.int32)
+        L.lib().pgfh_get_node_detect(self.h, L.ptr(a, C.c_int32))
+        return a
+
+    def dofs(self):
+        dof = np.zeros((self.dim, self.info.n_all_nodes), np.int32)
+        pres = np.zeros(max(self.info.n_pres, 1))
+        L.lib().pgfh_get_dofs(self.h, L.ptr(dof, C.c_int32), L.ptr(pres, C.c_double))
+        return dof, pres[: self.info.n_pres]
+
+    def enrich_edges(self):
+        n = self.info.n_all_nodes - self.info.n_nodes
+        a = np.zeros((max(n, 1), 3), np.int32)
+        L.lib().pgfh_get_enrich_edges(self.h, L.ptr(a, C.c_int32))
+        return a[:n]
+
+    def cut(self):
+        a = np.zeros((max(self.info.n_cut, 1), 128), np.int32)
+        L.lib().pgfh_get_cut(self.h, L.ptr(a, C.c_int32))
+        return a[: self.info.n_cut]
+
+    def pattern(self):
+        """(rowptr, rowsplit, colidx, color) built on the host exactly as pgf_create does."""
+        lib = L.lib()
+        n, nnz, nc = C.c_int64(), C.c_int64(), C.c_int32()
+        L.check(lib.pgfh_pattern_info(self.h, C.byref(n), C.byref(nnz), C.byref(nc)), host=True)
+        rp = np.zeros(n.value + 1, np.int64)
+        rs = np.zeros(n.value, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        col = np.zeros(self.info.n_elem, np.int32)
+        L.check(lib.pgfh_get_pattern(self.h, L.ptr(rp, C.c_int64), L.ptr(rs, C.c_int32), L.ptr(ci, C.c_int32),
+                                     L.ptr(col, C.c_int32)), host=True)
+        return rp, rs, ci, col
+
+    def desc(self):
+        d = L.MeshDesc()
+        L.check(L.lib().pgfh_mesh_desc(self.h, C.byref(d)), host=True)
+        return d
+
+
+class DeviceContext:
+    """One mesh resident on the GPU (pgf_create)."""
+
+    def __init__(self, prob):
+        self.prob = prob
+        self.lib = L.lib()
+        d = prob.desc()
+        h = C.c_void_p()
+        L.check(self.lib.pgf_create(C.byref(d), C.byref(h)), None)
+        self.h = h
+        self.csr = L.CsrInfo()
+        self.lib.pgf_csr_info_get(self.h, C.byref(self.csr))
+        dof, pres = prob.dofs()
+        self.pres_full = np.ascontiguousarray(pres)
+        L.check(self.lib.pgf_set_load(self.h, L.ptr(self.pres_full, C.c_double), None), self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pgf_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        L.check(rc, self.h)
+
+    def pattern(self):
+        n, nnz = self.csr.n_rows, self.csr.nnz
+        rp = np.zeros(n + 1, np.int64)
+        rs = np.zeros(n, np.int32)
+        ci = np.zeros(nnz, np.int32)
+        self._chk(self.lib.pgf_get_csr(self.h, L.ptr(rp, C.c_int64), L.ptr(rs, C.c_int32), L.ptr(ci, C.c_int32)))
+        return rp, rs, ci
+
+    def colors(self):
+        a = np.zeros(self.prob.info.n_elem, np.int32)
+        self._chk(self.lib.pgf_get_colors(self.h, L.ptr(a, C.c_int32)))
+        return a
+
+    def set_solution(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._chk(self.lib.pgf_set_solution(self.h, L.ptr(u, C.c_double)))
+
+    def solution(self):
+        u = np.zeros((self.prob.dim, self.prob.info.n_all_nodes))
+        self._chk(self.lib.pgf_get_solution(self.h, L.ptr(u, C.c_double)))
+        return u
+
+    def apply_load_fraction(self, frac):
+        self._chk(self.lib.pgf_apply_load_fraction(self.h, frac))
+
+    def assemble(self, flags=L.PGF_ASM_JAC | L.PGF_ASM_FUNC, dt=1.0):
+        self._chk(self.lib.pgf_assemble(self.h, dt, flags))
+
+    def matrix(self):
+        v = np.zeros(self.csr.nnz)
+        self._chk(self.lib.pgf_get_matrix(self.h, L.ptr(v, C.c_double)))
+        return v
+
+    def matrix32(self):
+        v = np.zeros(self.csr.nnz, np.float32)
+        self._chk(self.lib.pgf_get_matrix32(self.h, L.ptr(v, C.c_float)))
+        return v
+
+    def internal_force(self):
+        v = np.zeros(self.csr.n_rows)
+        self._chk(self.lib.pgf_get_internal_force(self.h, L.ptr(v, C.c_double)))
+        return v
+
+    def isv(self):
+        n = C.c_int64()
+        self.lib.pgf_n_cohesive_points(self.h, C.byref(n))
+        n = n.value
+        dt, dc, fl = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1), np.uint8)
+        self._chk(self.lib.pgf_get_isv(self.h, L.ptr(dt, C.c_double), L.ptr(dc, C.c_double), L.ptr(fl, C.c_uint8)))
+        return dt[:n], dc[:n], fl[:n]
+
+    def isv_commit(self):
+        self._chk(self.lib.pgf_isv_commit(self.h))
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.csr.n_free)
+        self._chk(self.lib.pgf_spmv(self.h, L.ptr(x, C.c_double), L.ptr(y, C.c_double)))
+        return y
+
+    def pcg(self, b, rtol=1e-10, maxit=20000, precision=64, x0=None):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.csr.n_free)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        info = L.SolveInfo()
+        self._chk(self.lib.pgf_pcg(self.h, L.ptr(b, C.c_double), L.ptr(x, C.c_double), L.ptr(x0, C.c_double),
+                                   rtol, maxit, precision, C.byref(info)))
+        return x, info
+
+    def newton_step(self, frac, rel_tol=1e-8, abs_tol=0.0, max_iter=40, precision=64, lin_rtol=1e-10,
+                    lin_maxit=50000, dt=1.0, use_guess=False):
+        o = L.NewtonOpts(rel_tol, abs_tol, max_iter, precision, lin_rtol, lin_maxit, int(use_guess), dt)
+        si = L.StepInfo()
+        rc = self.lib.pgf_newton_step(self.h, frac, C.byref(o), C.byref(si))
+        if rc not in (0, -5):
+            self._chk(rc)
+        return si
+
+    def time_assemble(self, flags, reps=10):
+        ms = C.c_double()
+        self._chk(self.lib.pgf_time_assemble(self.h, flags, reps, C.byref(ms)))
+        return ms.value
+
+    def time_spmv(self, precision=64, reps=20):
+        ms = C.c_double()
+        self._chk(self.lib.pgf_time_spmv(self.h, precision, reps, C.byref(ms)))
+        return ms.value
+
+    def solve_load_steps(self, T_final=1.0, max_time_step=0.05, min_time_step=1e-6, max_repeats=20, max_steps=0,
+                         callback=None, **newton):
+        nw = dict(rel_tol=1e-8, abs_tol=0.0, max_iter=40, precision=64, lin_rtol=1e-10, lin_maxit=50000,
+                  use_guess=0, dt=max_time_step)
+        nw.update(newton)
+        o = L.LoadOpts(T_final, max_time_step, min_time_step, max_repeats, max_steps,
+                       L.NewtonOpts(nw["rel_tol"], nw["abs_tol"], nw["max_iter"], nw["precision"], nw["lin_rtol"],
+                                    nw["lin_maxit"], int(nw["use_guess"]), nw["dt"]))
+        records = []
+
+        def _cb(user, step, t, dt, si):
+            s = si.contents
+            records.append(dict(step=step, t=t, dt=dt, newton=s.newton_iterations, pcg=s.pcg_iterations,
+                                res0=s.res0, res=s.res, ms=s.ms_total, ms_asm=s.ms_assemble, ms_solve=s.ms_solve))
+            if callback:
+                callback(records[-1])
+
+        cb = L.STEP_CB(_cb)
+        info = L.LoadInfo()
+        rc = self.lib.pgfh_solve_load_steps(self.prob.h, self.h, C.byref(o), cb, None, C.byref(info))
+        if rc != 0:
+            raise RuntimeError("load stepping failed (%d): %s" % (rc, self.lib.pgfh_last_error().decode()))
+        return info, records

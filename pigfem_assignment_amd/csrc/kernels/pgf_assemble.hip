@@ -1,0 +1,393 @@
+// Element assembly kernels (volumetric uncut, cut/IGFEM with cohesive points).
+//
+// Per quadrature point the math is the reference's ProblemUtilities fast
+// kernels: strain B*u (SmallStrainFast, src/Utilities.cpp:300-361), stress
+// D*eps with the isotropic zero structure (computeStress,
+// src/material_lei.cpp:232-261), K += w * B^T D B in the zero-skipping
+// SmallStrainKFast form (src/Utilities.cpp:196-296), P += w * B^T sigma
+// (SmallStrainInternalForceFast, :364-408), cohesive opening / stiffness /
+// force (cohesiveDeltaFast, cohesiveKFast, cohesiveInternalForceFast,
+// :465-531).  The scatter is Assembler::set_matrix / set_vec
+// (src/Assembler.cpp:811-850) made atomic-free by element colouring.
+#include "pgf_kernels.hpp"
+#include "pgf_device.hpp"
+
+namespace pgfk {
+
+// parts[k] = sum_l B_l,(a,da) D_lk for Voigt row k, the reference's part0..5
+template <int DIM>
+__device__ inline void k_parts(const double* g, int da, const double* D, double* parts) {
+    if (DIM == 2) {
+        if (da == 0)
+            for (int k = 0; k < 3; ++k) parts[k] = g[0] * D[0 * 3 + k] + g[1] * D[2 * 3 + k];
+        else
+            for (int k = 0; k < 3; ++k) parts[k] = g[1] * D[1 * 3 + k] + g[0] * D[2 * 3 + k];
+    } else {
+        if (da == 0)
+            for (int k = 0; k < 6; ++k) parts[k] = g[0] * D[0 * 6 + k] + g[2] * D[4 * 6 + k] + g[1] * D[5 * 6 + k];
+        else if (da == 1)
+            for (int k = 0; k < 6; ++k) parts[k] = g[1] * D[1 * 6 + k] + g[2] * D[3 * 6 + k] + g[0] * D[5 * 6 + k];
+        else
+            for (int k = 0; k < 6; ++k) parts[k] = g[2] * D[2 * 6 + k] + g[1] * D[3 * 6 + k] + g[0] * D[4 * 6 + k];
+    }
+}
+// K(row, (b,db)) from parts and column node gradient gb
+template <int DIM>
+__device__ inline double k_entry(const double* parts, const double* gb, int db) {
+    if (DIM == 2) return db == 0 ? parts[0] * gb[0] + parts[2] * gb[1] : parts[1] * gb[1] + parts[2] * gb[0];
+    if (db == 0) return parts[0] * gb[0] + parts[4] * gb[2] + parts[5] * gb[1];
+    if (db == 1) return parts[1] * gb[1] + parts[3] * gb[2] + parts[5] * gb[0];
+    return parts[2] * gb[2] + parts[3] * gb[1] + parts[4] * gb[0];
+}
+// strain = B u for nn nodes with gradients g[n*DIM+i] and element vector u[n*DIM+d]
+template <int DIM>
+__device__ inline void strain_of(int nn, const double* g, const double* u, double* eps) {
+    if (DIM == 2) {
+        eps[0] = eps[1] = eps[2] = 0.0;
+        for (int n = 0; n < nn; ++n) eps[0] += g[n * 2 + 0] * u[n * 2];
+        for (int n = 0; n < nn; ++n) eps[1] += g[n * 2 + 1] * u[n * 2 + 1];
+        for (int n = 0; n < nn; ++n) eps[2] += g[n * 2 + 1] * u[n * 2] + g[n * 2 + 0] * u[n * 2 + 1];
+    } else {
+        for (int k = 0; k < 6; ++k) eps[k] = 0.0;
+        for (int n = 0; n < nn; ++n) eps[0] += g[n * 3 + 0] * u[n * 3];
+        for (int n = 0; n < nn; ++n) eps[1] += g[n * 3 + 1] * u[n * 3 + 1];
+        for (int n = 0; n < nn; ++n) eps[2] += g[n * 3 + 2] * u[n * 3 + 2];
+        for (int n = 0; n < nn; ++n) eps[3] += g[n * 3 + 2] * u[n * 3 + 1] + g[n * 3 + 1] * u[n * 3 + 2];
+        for (int n = 0; n < nn; ++n) eps[4] += g[n * 3 + 2] * u[n * 3] + g[n * 3 + 0] * u[n * 3 + 2];
+        for (int n = 0; n < nn; ++n) eps[5] += g[n * 3 + 1] * u[n * 3] + g[n * 3 + 0] * u[n * 3 + 1];
+    }
+}
+template <int DIM>
+__device__ inline void stress_of(const double* D, const double* e, double* s) {
+    if (DIM == 2) {
+        s[0] = D[0] * e[0] + D[1] * e[1];
+        s[1] = D[3] * e[0] + D[4] * e[1];
+        s[2] = D[8] * e[2];
+    } else {
+        s[0] = D[0] * e[0] + D[1] * e[1] + D[2] * e[2];
+        s[1] = D[6] * e[0] + D[7] * e[1] + D[8] * e[2];
+        s[2] = D[12] * e[0] + D[13] * e[1] + D[14] * e[2];
+        s[3] = D[21] * e[3];
+        s[4] = D[28] * e[4];
+        s[5] = D[35] * e[5];
+    }
+}
+// B^T sigma for one node (SmallStrainInternalForceFast)
+template <int DIM>
+__device__ inline double force_entry(const double* g, int d, const double* s) {
+    if (DIM == 2) return d == 0 ? g[0] * s[0] + g[1] * s[2] : g[1] * s[1] + g[0] * s[2];
+    if (d == 0) return g[0] * s[0] + g[2] * s[4] + g[1] * s[5];
+    if (d == 1) return g[1] * s[1] + g[2] * s[3] + g[0] * s[5];
+    return g[2] * s[2] + g[1] * s[3] + g[0] * s[4];
+}
+
+// ---------------------------------------------------------------------------
+// Uncut elements: one thread per element.
+template <int DIM, bool DO_K, bool DO_F, typename KT>
+__global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t end) {
+    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    constexpr int S = DIM == 2 ? 3 : 6;
+    constexpr int NQ = DIM == 2 ? 4 : 1;  // linear tets: one merged point
+    int nd[4];
+    double X[4][DIM];
+    for (int k = 0; k < 4; ++k) {
+        nd[k] = a.ue_conn[k * a.n_uncut + p];
+        for (int i = 0; i < DIM; ++i) X[k][i] = a.X[i * a.n_all + nd[k]];
+    }
+    const int mat = a.ue_mat[p];
+    double D[S * S];
+    for (int k = 0; k < S * S; ++k) D[k] = a.Dmat[mat * 36 + k];
+    double g[NQ][4 * DIM], w[NQ];
+    if (DIM == 2) {
+        const double r0 = 0.57735026918962576450914878050196;
+        for (int q = 0; q < 4; ++q) {
+            double r = (q % 2 == 0) ? -r0 : r0, s = (q / 2 == 0) ? -r0 : r0;
+            double dr[4][2] = {{-0.25 * (1.0 - s), -0.25 * (1.0 - r)},
+                               {0.25 * (1.0 - s), -0.25 * (1.0 + r)},
+                               {0.25 * (1.0 + s), 0.25 * (1.0 + r)},
+                               {-0.25 * (1.0 + s), 0.25 * (1.0 - r)}};
+            double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+            for (int n = 0; n < 4; ++n) {
+                J00 += dr[n][0] * X[n][0]; J01 += dr[n][0] * X[n][1];
+                J10 += dr[n][1] * X[n][0]; J11 += dr[n][1] * X[n][1];
+            }
+            double det = J00 * J11 - J01 * J10;
+            double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
+            for (int n = 0; n < 4; ++n) {
+                g[q][n * 2 + 0] = i00 * dr[n][0] + i01 * dr[n][1];
+                g[q][n * 2 + 1] = i10 * dr[n][0] + i11 * dr[n][1];
+            }
+            w[q] = det * 1.0;
+        }
+    } else {
+        // dN/dr constant: rows of the tet map are x1-x0, x2-x0, x3-x0
+        double J[3][3];
+        for (int i = 0; i < 3; ++i) {
+            J[0][i] = X[1][i] - X[0][i];
+            J[1][i] = X[2][i] - X[0][i];
+            J[2][i] = X[3][i] - X[0][i];
+        }
+        double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        double inv[3][3];
+        inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+        inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        const double dr[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        for (int n = 0; n < 4; ++n)
+            for (int i = 0; i < 3; ++i) g[0][n * 3 + i] = inv[i][0] * dr[n][0] + inv[i][1] * dr[n][1] + inv[i][2] * dr[n][2];
+        // five-point rule, weights -0.8, 4 x 0.45 (sum 1), all points share B
+        double ws = det * -0.8;
+        for (int q = 0; q < 4; ++q) ws += det * 0.45;
+        w[0] = ws * a.tet_scale;
+    }
+    int row_dof[4][DIM];
+    for (int n = 0; n < 4; ++n)
+        for (int d = 0; d < DIM; ++d) row_dof[n][d] = a.dof[d * a.n_all + nd[n]];
+    if (DO_F) {
+        double u[4 * DIM];
+        for (int n = 0; n < 4; ++n)
+            for (int d = 0; d < DIM; ++d) u[n * DIM + d] = a.U[d * a.n_all + nd[n]];
+        double Pe[4 * DIM];
+        for (int i = 0; i < 4 * DIM; ++i) Pe[i] = 0.0;
+        for (int q = 0; q < NQ; ++q) {
+            double eps[S], sig[S];
+            strain_of<DIM>(4, g[q], u, eps);
+            stress_of<DIM>(D, eps, sig);
+            for (int n = 0; n < 4; ++n)
+                for (int d = 0; d < DIM; ++d) Pe[n * DIM + d] += w[q] * force_entry<DIM>(&g[q][n * DIM], d, sig);
+        }
+        for (int n = 0; n < 4; ++n)
+            for (int d = 0; d < DIM; ++d) {
+                int r = row_dof[n][d];
+                if (r < a.n_free || a.all_rows) a.P[r] += Pe[n * DIM + d];
+            }
+    }
+    if (DO_K) {
+        uint32_t offw[8];
+        for (int k = 0; k < 8; ++k) offw[k] = a.ue_off[k * a.n_uncut + p];
+        for (int an = 0; an < 4; ++an)
+            for (int da = 0; da < DIM; ++da) {
+                const int r = row_dof[an][da];
+                if (r >= a.n_free && !a.all_rows) continue;
+                double Kr[4 * DIM];
+                for (int j = 0; j < 4 * DIM; ++j) Kr[j] = 0.0;
+                for (int q = 0; q < NQ; ++q) {
+                    double parts[S];
+                    k_parts<DIM>(&g[q][an * DIM], da, D, parts);
+                    for (int bn = 0; bn < 4; ++bn)
+                        for (int db = 0; db < DIM; ++db)
+                            Kr[bn * DIM + db] += w[q] * k_entry<DIM>(parts, &g[q][bn * DIM], db);
+                }
+                const int64_t base = a.rowptr[r];
+                for (int bn = 0; bn < 4; ++bn) {
+                    const int k = (an * 4 + bn) * 2;
+                    const int of = (offw[k >> 2] >> ((k & 3) * 8)) & 0xff;
+                    const int op = (offw[(k + 1) >> 2] >> (((k + 1) & 3) * 8)) & 0xff;
+                    int cf = 0, cp = 0;
+                    for (int db = 0; db < DIM; ++db) {
+                        const int c = row_dof[bn][db];
+                        const int64_t pos = c < a.n_free ? base + of + cf++ : base + op + cp++;
+                        if (sizeof(KT) == 8) a.K[pos] += Kr[bn * DIM + db];
+                        else a.K32[pos] += (float)Kr[bn * DIM + db];
+                    }
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cut elements: one wavefront per element, element matrix in LDS.
+template <int DIM, bool DO_K, bool DO_F, typename KT>
+__global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
+    const int64_t li = begin + blockIdx.x;
+    if (li >= end) return;
+    const int c = a.ce_list[li];
+    const int tid = threadIdx.x;
+    constexpr int S = DIM == 2 ? 3 : 6;
+    const int nt = a.ce_nt[c];
+    const int M = nt * DIM;
+    __shared__ int s_node[MAX_NT];
+    __shared__ int s_dof[MAX_M];
+    __shared__ double s_u[MAX_M];
+    __shared__ double s_K[MAX_M * MAX_M];
+    __shared__ double s_P[MAX_M];
+    __shared__ double s_g[MAX_NT * 3];
+    __shared__ double s_DB[6 * MAX_M];
+    __shared__ double s_sig[6];
+    __shared__ double s_D[36];
+    __shared__ double s_RDR[9], s_tR[3], s_N[4], s_w;
+    __shared__ int s_map[8], s_nh;
+    if (tid < nt) s_node[tid] = a.ce_nodes[c * 16 + tid];
+    __syncthreads();
+    for (int i = tid; i < M; i += WAVE) {
+        s_dof[i] = a.dof[(i % DIM) * a.n_all + s_node[i / DIM]];
+        s_u[i] = DO_F ? a.U[(i % DIM) * a.n_all + s_node[i / DIM]] : 0.0;
+        s_P[i] = 0.0;
+    }
+    for (int i = tid; i < M * M; i += WAVE) s_K[i] = 0.0;
+    __syncthreads();
+    // volumetric points
+    for (int64_t pt = a.vpt_ptr[c]; pt < a.vpt_ptr[c + 1]; ++pt) {
+        const double* gp = a.vpt_grad + a.vpt_grad_off[pt];
+        for (int i = tid; i < M; i += WAVE) s_g[i] = gp[i];
+        const int mat = a.vpt_mat[pt];
+        if (tid < S * S) s_D[tid] = a.Dmat[mat * 36 + tid];
+        const double w = a.vpt_w[pt];
+        __syncthreads();
+        if (DO_F && tid == 0) {
+            double eps[S];
+            strain_of<DIM>(nt, s_g, s_u, eps);
+            stress_of<DIM>(s_D, eps, s_sig);
+        }
+        if (DO_K) {
+            // parts for every element row (a, da): s_DB[i*S + k]
+            for (int i = tid; i < M; i += WAVE) k_parts<DIM>(&s_g[(i / DIM) * DIM], i % DIM, s_D, &s_DB[i * S]);
+        }
+        __syncthreads();
+        if (DO_K)
+            for (int e = tid; e < M * M; e += WAVE) {
+                const int i = e / M, j = e % M;
+                s_K[e] += w * k_entry<DIM>(&s_DB[i * S], &s_g[(j / DIM) * DIM], j % DIM);
+            }
+        if (DO_F)
+            for (int i = tid; i < M; i += WAVE) s_P[i] += w * force_entry<DIM>(&s_g[(i / DIM) * DIM], i % DIM, s_sig);
+        __syncthreads();
+    }
+    // cohesive points
+    for (int64_t q = a.cpt_ptr[c]; q < a.cpt_ptr[c + 1]; ++q) {
+        if (tid == 0) {
+            const int nh = a.cpt_nh[q];
+            double N[4], R[9], dl[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
+            int map[8];
+            for (int k = 0; k < 4; ++k) N[k] = a.cpt_N[q * 4 + k];
+            for (int k = 0; k < 9; ++k) R[k] = a.cpt_R[q * 9 + k];
+            for (int k = 0; k < 2 * nh; ++k) map[k] = a.cpt_nodes[q * 8 + k];
+            for (int d = 0; d < DIM; ++d)
+                for (int n = 0; n < nh; ++n) dl[d] += N[n] * (s_u[map[nh + n] * DIM + d] - s_u[map[n] * DIM + d]);
+            for (int i = 0; i < DIM; ++i)
+                for (int j = 0; j < DIM; ++j) delta[i] += R[i * DIM + j] * dl[j];
+            double trac[3], Dc[9], dnew;
+            uint8_t fl;
+            coh_law(a.laws[a.cpt_law[q]], DIM, delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
+            a.dmax_trial[q] = dnew;
+            a.flags[q] = fl;
+            // R^T D R: (R^T D) then * R
+            double RtD[9];
+            for (int i = 0; i < DIM; ++i)
+                for (int j = 0; j < DIM; ++j) {
+                    double s = 0.0;
+                    for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * Dc[k * DIM + j];
+                    RtD[i * DIM + j] = s;
+                }
+            for (int i = 0; i < DIM; ++i)
+                for (int j = 0; j < DIM; ++j) {
+                    double s = 0.0;
+                    for (int k = 0; k < DIM; ++k) s += RtD[i * DIM + k] * R[k * DIM + j];
+                    s_RDR[i * DIM + j] = s;
+                }
+            for (int i = 0; i < DIM; ++i) {
+                double s = 0.0;
+                for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * trac[k];
+                s_tR[i] = s;
+            }
+            for (int k = 0; k < 4; ++k) s_N[k] = N[k];
+            for (int k = 0; k < 8; ++k) s_map[k] = k < 2 * nh ? map[k] : 0;
+            s_nh = nh;
+            s_w = a.cpt_w[q];
+        }
+        __syncthreads();
+        const int nh = s_nh, mc = 2 * nh * DIM;
+        const double w = s_w;
+        if (DO_K)
+            for (int e = tid; e < mc * mc; e += WAVE) {
+                const int i = e / mc, j = e % mc;
+                const int n1 = i / DIM, d1 = i % DIM, n2 = j / DIM, d2 = j % DIM;
+                const double v = s_N[n1 % nh] * s_N[n2 % nh] * s_RDR[d1 * DIM + d2];
+                const double sgn = ((n1 < nh) == (n2 < nh)) ? 1.0 : -1.0;
+                s_K[(s_map[n1] * DIM + d1) * M + s_map[n2] * DIM + d2] += w * (sgn * v);
+            }
+        if (DO_F)
+            for (int i = tid; i < mc; i += WAVE) {
+                const int n = i / DIM, d = i % DIM;
+                const double v = s_N[n % nh] * s_tR[d];
+                s_P[s_map[n] * DIM + d] += w * (n < nh ? -v : v);
+            }
+        __syncthreads();
+    }
+    // scatter
+    const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
+    if (DO_F)
+        for (int i = tid; i < M; i += WAVE) {
+            const int r = s_dof[i];
+            if (r < a.n_free || a.all_rows) a.P[r] += s_P[i];
+        }
+    if (DO_K)
+        for (int e = tid; e < M * M; e += WAVE) {
+            const int i = e / M, j = e % M;
+            const int r = s_dof[i];
+            if (r >= a.n_free && !a.all_rows) continue;
+            const int an = i / DIM, bn = j / DIM, db = j % DIM;
+            const int of = off[(an * nt + bn) * 2], op = off[(an * nt + bn) * 2 + 1];
+            // rank of db among node bn's free / prescribed dofs
+            int cf = 0, cp = 0;
+            for (int k = 0; k < db; ++k) { if (s_dof[bn * DIM + k] < a.n_free) ++cf; else ++cp; }
+            const int cdof = s_dof[j];
+            const int64_t pos = cdof < a.n_free ? a.rowptr[r] + of + cf : a.rowptr[r] + op + cp;
+            if (sizeof(KT) == 8) a.K[pos] += s_K[e];
+            else a.K32[pos] += (float)s_K[e];
+        }
+}
+
+}  // namespace pgfk
+
+// ---------------------------------------------------------------------------
+// Launch one full assembly (all colours) on the context stream.
+namespace pgfd {
+
+template <int DIM, bool DO_K, bool DO_F, typename KT>
+static void launch_colors(const Ctx& c, const pgfk::AsmArgs& a) {
+    for (int col = 0; col < c.n_colors; ++col) {
+        int64_t b = c.ue_color_ptr[col], e = c.ue_color_ptr[col + 1];
+        if (e > b) {
+            int64_t nb = (e - b + 255) / 256;
+            hipLaunchKernelGGL((pgfk::k_uncut<DIM, DO_K, DO_F, KT>), dim3((unsigned)nb), dim3(256), 0, c.stream, a, b, e);
+        }
+        int64_t cb = c.ce_color_ptr[col], ce = c.ce_color_ptr[col + 1];
+        if (ce > cb)
+            hipLaunchKernelGGL((pgfk::k_cut<DIM, DO_K, DO_F, KT>), dim3((unsigned)(ce - cb)), dim3(pgfk::WAVE), 0, c.stream, a,
+                               cb, ce);
+    }
+}
+
+template <int DIM>
+static void launch_dim(const Ctx& c, const pgfk::AsmArgs& a, bool jac, bool func, bool fp32) {
+    if (jac && func && !fp32) launch_colors<DIM, true, true, double>(c, a);
+    else if (jac && !fp32) launch_colors<DIM, true, false, double>(c, a);
+    else if (jac && fp32) launch_colors<DIM, true, false, float>(c, a);
+    if (func && (fp32 || !jac)) launch_colors<DIM, false, true, double>(c, a);
+}
+
+void enqueue_assembly(Ctx& c, double dt, uint32_t flags) {
+    const bool jac = flags & 2u, func = flags & 1u, fp32 = flags & 4u;
+    const bool all_rows = flags & 8u;
+    pgfk::AsmArgs a = c.asm_args();
+    a.dt = dt;
+    a.all_rows = all_rows ? 1 : 0;
+    if (jac) {
+        if (fp32) fill_async(c, c.K32, c.nnz, 0.0f);
+        else fill_async(c, c.K, c.nnz, 0.0);
+    }
+    if (func) fill_async(c, c.P, c.n_rows, 0.0);
+    if (c.dim == 2) launch_dim<2>(c, a, jac, func, fp32);
+    else launch_dim<3>(c, a, jac, func, fp32);
+}
+
+}  // namespace pgfd

@@ -1,0 +1,924 @@
+// Device context, Krylov kernels, Newton step and the pgf_* C-ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+#include "../../../include/pgfem.h"
+#include "../host/pgf_dofs.hpp"
+#include "pgf_device.hpp"
+
+namespace pgfd {
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            throw pgf::Error(pgf::PGF_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static std::string g_last_error;
+
+template <typename T>
+static T* dalloc(int64_t n) {
+    T* p = nullptr;
+    if (n <= 0) n = 1;
+    HIPCHK(hipMalloc((void**)&p, sizeof(T) * (size_t)n));
+    return p;
+}
+template <typename T>
+static T* dupload(const T* h, int64_t n) {
+    T* p = dalloc<T>(n);
+    if (n > 0 && h) HIPCHK(hipMemcpy(p, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice));
+    return p;
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+template <typename T>
+void fill_async(Ctx& c, T* p, int64_t n, T v) {
+    if (n <= 0) return;
+    int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)nb), dim3(256), 0, c.stream, p, n, v);
+}
+template void fill_async<double>(Ctx&, double*, int64_t, double);
+template void fill_async<float>(Ctx&, float*, int64_t, float);
+
+pgfk::AsmArgs Ctx::asm_args() const {
+    pgfk::AsmArgs a{};
+    a.n_all = n_all; a.n_free = n_free; a.n_rows = n_rows; a.n_uncut = n_uncut;
+    a.X = X; a.dof = dof; a.U = U;
+    a.ue_conn = ue_conn; a.ue_mat = ue_mat; a.ue_off = ue_off; a.Dmat = Dmat; a.tet_scale = tet_scale;
+    a.ce_list = ce_list; a.ce_nodes = ce_nodes; a.ce_nt = ce_nt;
+    a.vpt_ptr = vpt_ptr; a.vpt_w = vpt_w; a.vpt_mat = vpt_mat; a.vpt_grad_off = vpt_grad_off; a.vpt_grad = vpt_grad;
+    a.cpt_ptr = cpt_ptr; a.cpt_w = cpt_w; a.cpt_N = cpt_N; a.cpt_R = cpt_R; a.cpt_nodes = cpt_nodes; a.cpt_nh = cpt_nh;
+    a.cpt_law = cpt_law; a.laws = laws; a.ce_off = ce_off; a.ce_off_ptr = ce_off_ptr;
+    a.dmax_conv = dmax_conv; a.dmax_trial = dmax_trial; a.flags = flags;
+    a.rowptr = rowptr; a.K = K; a.K32 = K32; a.P = P;
+    return a;
+}
+
+Ctx::~Ctx() {
+    void* ptrs[] = {X, dof, U, ue_conn, ue_mat, ue_off, Dmat, ce_list, ce_nodes, ce_nt, vpt_ptr, vpt_grad_off, cpt_ptr,
+                    ce_off_ptr, vpt_w, vpt_grad, cpt_w, cpt_N, cpt_R, vpt_mat, cpt_law, cpt_nodes, cpt_nh, ce_off, laws,
+                    dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
+                    upres_full, r, x, z, p0, p1, q, dinv, guess, res, r32, x32, z32, p032, p132, q32, dinv32, partial,
+                    scal, counter, done};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h_scal) (void)hipHostFree(h_scal);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ---------------------------------------------------------------------------
+// reductions: wave64 shuffles, then LDS across the block's waves
+template <int NV>
+__device__ inline void block_reduce(double* v) {
+    __shared__ double sh[NV][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < NV; ++k)
+        for (int m = 32; m >= 1; m >>= 1) v[k] += __shfl_xor(v[k], m, 64);
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < NV; ++k) sh[k][wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += sh[k][w];
+            v[k] = s;
+        }
+}
+
+// Publish a block's partial sums; returns true in the one block that arrives
+// last, after which it may read every block's partials (fixed-order sum).
+template <int NV>
+__device__ inline bool last_block(double* v, double* partial, unsigned* counter) {
+    __shared__ bool s_last;
+    block_reduce<NV>(v);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k) partial[k * gridDim.x + blockIdx.x] = v[k];
+        __threadfence();
+        unsigned t = atomicAdd(counter, 1u);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) s += partial[k * gridDim.x + i];
+        v[k] = s;
+    }
+    block_reduce<NV>(v);
+    return true;
+}
+
+// scalar slots of the PCG recurrence
+enum { S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_RR0, S_TOL2, S_ITER, S_MAXIT, S_BB, S_NSCAL = 16 };
+
+constexpr int RED_BLOCKS = 1024;
+
+// ---------------------------------------------------------------------------
+// SpMV on the free-free block: y = K_ff x, G lanes per row.
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
+                                              const T* __restrict__ A, const T* __restrict__ x, T* __restrict__ y) {
+    const int lane = threadIdx.x % G;
+    for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; row < n;
+         row += (int64_t)gridDim.x * blockDim.x / G) {
+        const int32_t b = rowptr[row], e = b + rowsplit[row];
+        double acc = 0.0;
+        for (int32_t j = b + lane; j < e; j += G) acc += (double)A[j] * (double)x[col[j]];
+        for (int m = G / 2; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, G);
+        if (lane == 0) y[row] = (T)acc;
+    }
+}
+
+// PCG step A: p_new = z + beta p_old (own rows), q = A p_new (rows gathered on
+// the fly from z and p_old), alpha = rz / (p.q) in the last block.
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
+                                                 const T* __restrict__ A, const T* __restrict__ z,
+                                                 const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
+                                                 double* scal, double* partial, unsigned* counter, const int* done) {
+    if (*done) return;
+    const double beta = scal[S_BETA];
+    const int lane = threadIdx.x % G;
+    double dot = 0.0;
+    for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; row < n;
+         row += (int64_t)gridDim.x * blockDim.x / G) {
+        const int32_t b = rowptr[row], e = b + rowsplit[row];
+        double acc = 0.0;
+        for (int32_t j = b + lane; j < e; j += G) {
+            const int32_t c = col[j];
+            acc += (double)A[j] * ((double)z[c] + beta * (double)pold[c]);
+        }
+        for (int m = G / 2; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, G);
+        if (lane == 0) {
+            const double pn = (double)z[row] + beta * (double)pold[row];
+            pnew[row] = (T)pn;
+            q[row] = (T)acc;
+            dot += pn * acc;
+        }
+    }
+    double v[1] = {dot};
+    if (last_block<1>(v, partial, counter) && threadIdx.x == 0) {
+        const double pq = v[0];
+        scal[S_ALPHA] = (pq != 0.0 && std::isfinite(pq)) ? scal[S_RZ] / pq : 0.0;
+        *counter = 0;
+    }
+}
+
+// PCG step B: x += alpha p, r -= alpha q, z = D^{-1} r, then beta and the
+// convergence flag in the last block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_update(int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                   const T* __restrict__ p, const T* __restrict__ q,
+                                                   const T* __restrict__ dinv, double* scal, double* partial,
+                                                   unsigned* counter, int* done, double* hist) {
+    if (*done) return;
+    const double alpha = scal[S_ALPHA];
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double xi = (double)x[i] + alpha * (double)p[i];
+        const double ri = (double)r[i] - alpha * (double)q[i];
+        const double zi = (double)dinv[i] * ri;
+        x[i] = (T)xi; r[i] = (T)ri; z[i] = (T)zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    double v[2] = {rz, rr};
+    if (last_block<2>(v, partial, counter + 1) && threadIdx.x == 0) {
+        const double rz_old = scal[S_RZ];
+        scal[S_BETA] = rz_old != 0.0 ? v[0] / rz_old : 0.0;
+        scal[S_RZ] = v[0];
+        scal[S_RR] = v[1];
+        const double it = scal[S_ITER] + 1.0;
+        scal[S_ITER] = it;
+        if (hist) hist[(int)it] = v[1];
+        if (!(v[1] > scal[S_TOL2] * scal[S_RR0]) || it >= scal[S_MAXIT] || !std::isfinite(v[1])) *done = 1;
+        counter[1] = 0;
+    }
+}
+
+// PCG start: r = b - Ax (Ax precomputed in q, or zero), z = D^{-1} r, p = 0.
+template <typename T, typename TB>
+__global__ __launch_bounds__(256) void k_cg_init(int64_t n, const TB* __restrict__ b, T* __restrict__ r,
+                                                 T* __restrict__ z, T* __restrict__ p0, T* __restrict__ p1,
+                                                 const T* __restrict__ Ax, const T* __restrict__ dinv, double* scal,
+                                                 double* partial, unsigned* counter, int* done, double tol2,
+                                                 double maxit) {
+    double rz = 0.0, rr = 0.0, bb = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double bi = (double)b[i];
+        const double ri = Ax ? bi - (double)Ax[i] : bi;
+        const double zi = (double)dinv[i] * ri;
+        r[i] = (T)ri; z[i] = (T)zi; p0[i] = (T)0; p1[i] = (T)0;
+        rz += ri * zi; rr += ri * ri; bb += bi * bi;
+    }
+    double v[3] = {rz, rr, bb};
+    if (last_block<3>(v, partial, counter + 2) && threadIdx.x == 0) {
+        scal[S_RZ] = v[0]; scal[S_RR] = v[1]; scal[S_BB] = v[2];
+        // relative to ||b|| (KSP's default preconditioned/unpreconditioned choice is ||b||)
+        scal[S_RR0] = v[2];
+        scal[S_TOL2] = tol2; scal[S_BETA] = 0.0; scal[S_ALPHA] = 0.0; scal[S_ITER] = 0.0; scal[S_MAXIT] = maxit;
+        *done = (!(v[1] > tol2 * v[2]) || maxit <= 0) ? 1 : 0;
+        counter[2] = 0;
+    }
+}
+
+template <typename T>
+__global__ void k_diag_inv(int64_t n, const int32_t* rowptr, const int32_t* rowsplit, const int32_t* col,
+                           const T* A, T* dinv) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = rowptr[r], e = b + rowsplit[r];
+        double d = 0.0;
+        for (int32_t j = b; j < e; ++j)
+            if (col[j] == r) { d = (double)A[j]; break; }
+        dinv[r] = (T)(d != 0.0 ? 1.0 / d : 1.0);
+    }
+}
+
+// ---- Newton helpers ----
+__global__ void k_apply_frac(int64_t n, const int64_t* slot, const double* upf, double frac, double* U) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        U[slot[i]] = frac * upf[i];
+}
+// r = frac*Fext - P_int (free rows) and ||r||^2 into scal[slot]
+__global__ __launch_bounds__(256) void k_residual(int64_t n, const double* Fext, double frac, const double* P,
+                                                  double* r, double* partial, unsigned* counter, double* out) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = frac * Fext[i] - P[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    double v[1] = {s};
+    if (last_block<1>(v, partial, counter) && threadIdx.x == 0) {
+        *out = v[0];
+        *counter = 0;
+    }
+}
+template <typename T>
+__global__ void k_update_U(int64_t n, const int64_t* slot, const T* dx, double* U) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        U[slot[i]] += (double)dx[i];
+}
+__global__ void k_to_f32(int64_t n, const double* a, float* b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+__global__ void k_copy(int64_t n, const double* a, double* b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+static unsigned grid_for(int64_t n, int per_block = 256) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + per_block - 1) / per_block, RED_BLOCKS));
+}
+
+template <typename T>
+static void enqueue_spmv(Ctx& c, const T* A, const T* x, T* y) {
+    const unsigned nb = (unsigned)std::min<int64_t>((c.n_free * 16 + 255) / 256, 65536);
+    if (c.dim == 3) hipLaunchKernelGGL((k_spmv<T, 16>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
+    else hipLaunchKernelGGL((k_spmv<T, 8>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
+}
+
+// Enqueue a whole PCG solve in chunks; the device flag stops the recurrence,
+// the host only polls the pinned scalars between chunks.
+template <typename T>
+static int run_pcg(Ctx& c, const T* A, const void* b, bool b_is_double, T* x, T* r, T* z, T* p0, T* p1, T* q,
+                   T* dinv, bool x0, double rtol, int maxit, pgf_solve_info* info) {
+    const int64_t n = c.n_free;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c.stream));
+    hipLaunchKernelGGL((k_diag_inv<T>), dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx, A, dinv);
+    if (x0) enqueue_spmv<T>(c, A, x, q);
+    else fill_async<T>(c, x, n, (T)0);
+    const unsigned gi = grid_for(n);
+    if (b_is_double)
+        hipLaunchKernelGGL((k_cg_init<T, double>), dim3(gi), dim3(256), 0, c.stream, n, (const double*)b, r, z, p0, p1,
+                           x0 ? (const T*)q : nullptr, (const T*)dinv, c.scal, c.partial, c.counter, c.done, rtol * rtol,
+                           (double)maxit);
+    else
+        hipLaunchKernelGGL((k_cg_init<T, float>), dim3(gi), dim3(256), 0, c.stream, n, (const float*)b, r, z, p0, p1,
+                           x0 ? (const T*)q : nullptr, (const T*)dinv, c.scal, c.partial, c.counter, c.done, rtol * rtol,
+                           (double)maxit);
+    const int G = c.dim == 3 ? 16 : 8;
+    const unsigned ga = (unsigned)std::min<int64_t>((n * G + 255) / 256, RED_BLOCKS * 2);
+    const unsigned gu = grid_for(n);
+    const int chunk = 32;
+    int it = 0;
+    bool stop = false;
+    while (!stop) {
+        for (int k = 0; k < chunk; ++k, ++it) {
+            T* pold = (it % 2 == 0) ? p0 : p1;
+            T* pnew = (it % 2 == 0) ? p1 : p0;
+            if (G == 16)
+                hipLaunchKernelGGL((k_cg_spmv<T, 16>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
+                                   A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
+            else
+                hipLaunchKernelGGL((k_cg_spmv<T, 8>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
+                                   A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
+            hipLaunchKernelGGL((k_cg_update<T>), dim3(gu), dim3(256), 0, c.stream, n, x, r, z, pnew, q, dinv, c.scal,
+                               c.partial, c.counter, c.done, (double*)nullptr);
+        }
+        HIPCHK(hipMemcpyAsync(c.h_scal, c.scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipMemcpyAsync(c.h_scal + S_NSCAL, c.done, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        int dn;
+        std::memcpy(&dn, c.h_scal + S_NSCAL, sizeof(int));
+        if (dn || it >= maxit) stop = true;
+    }
+    HIPCHK(hipEventRecord(e1, c.stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double rr = c.h_scal[S_RR], rr0 = c.h_scal[S_RR0];
+    if (info) {
+        info->iterations = (int)c.h_scal[S_ITER];
+        info->rel_residual = rr0 > 0 ? std::sqrt(rr / rr0) : 0.0;
+        info->converged = rr0 == 0 || rr <= rtol * rtol * rr0;
+        info->ms = ms;
+    }
+    // the last iterate's p lives in the buffer written by the final step A
+    return 0;
+}
+
+}  // namespace pgfd
+
+using pgfd::Ctx;
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+namespace {
+int fail_ctx(Ctx* c, const pgf::Error& e) {
+    if (c) c->err = e.what();
+    pgfd::g_last_error = e.what();
+    return e.code;
+}
+int fail_ctx(Ctx* c, const std::exception& e) {
+    if (c) c->err = e.what();
+    pgfd::g_last_error = e.what();
+    return PGF_EARG;
+}
+template <typename F>
+int guard(Ctx* c, F&& f) {
+    try {
+        return f();
+    } catch (const pgf::Error& e) {
+        return fail_ctx(c, e);
+    } catch (const std::exception& e) {
+        return fail_ctx(c, e);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+struct pgf_ctx : public Ctx {};
+
+PGF_API const char* pgf_last_error(const pgf_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : pgfd::g_last_error.c_str();
+}
+
+PGF_API int pgf_device_info(char* name, int32_t name_len, int32_t* n_cu, double* hbm_gb) {
+    return guard(nullptr, [&]() -> int {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        HIPCHK(hipGetDeviceProperties(&p, dev));
+        if (name && name_len > 0) {
+            std::snprintf(name, (size_t)name_len, "%s %s", p.name, p.gcnArchName);
+        }
+        if (n_cu) *n_cu = p.multiProcessorCount;
+        if (hbm_gb) *hbm_gb = (double)p.totalGlobalMem / 1e9;
+        return 0;
+    });
+}
+
+PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
+    pgf_ctx* c = nullptr;
+    try {
+        if (!d || !out) throw pgf::Error(PGF_EARG, "null descriptor");
+        if (d->dim != 2 && d->dim != 3) throw pgf::Error(PGF_EARG, "dim must be 2 or 3");
+        if ((d->dim == 2) != (d->elem_kind == PGF_QUAD4)) throw pgf::Error(PGF_EARG, "element kind does not match dim");
+        if (d->n_mat <= 0 || d->n_mat > 255) throw pgf::Error(PGF_EARG, "n_mat must be in 1..255");
+        c = new pgf_ctx();
+        c->dim = d->dim;
+        c->kind = d->elem_kind;
+        c->n_nodes = d->n_nodes;
+        c->n_all = d->n_all_nodes;
+        c->n_elem = d->n_elem;
+        c->n_cut = d->n_cut;
+        c->n_free = d->n_free;
+        c->n_pres = d->n_pres;
+        c->n_rows = d->n_free + d->n_pres;
+        c->n_mat = d->n_mat;
+        // ---- host pattern, offsets and colours ----
+        pgf::MeshView mv;
+        mv.dim = d->dim; mv.n_nodes = d->n_nodes; mv.n_all = d->n_all_nodes; mv.n_elem = d->n_elem; mv.n_cut = d->n_cut;
+        mv.conn = d->conn; mv.cut_elem = d->cut_elem; mv.cut_n_enrich = d->cut_n_enrich; mv.cut_enrich = d->cut_enrich;
+        for (int64_t e = 0; e < d->n_elem * 4; ++e)
+            if (d->conn[e] < 0 || d->conn[e] >= d->n_nodes) throw pgf::Error(PGF_EARG, "connectivity out of range");
+        for (int64_t k = 0; k < d->n_cut * 16; ++k)
+            if (d->cut_enrich[k] < 0 || d->n_nodes + d->cut_enrich[k] >= d->n_all_nodes)
+                if (k % 16 < d->cut_n_enrich[k / 16]) throw pgf::Error(PGF_EARG, "enrichment node out of range");
+        mv.finalize();
+        pgf::Pattern pat;
+        pat.build(mv, d->dof, d->n_free, d->n_pres);
+        if (pat.nnz >= INT32_MAX) throw pgf::Error(PGF_ELIMIT, "nnz exceeds int32 range");
+        c->nnz = pat.nnz;
+        c->n_colors = pat.n_colors;
+        c->h_rowptr = pat.rowptr;
+        c->h_rowsplit = pat.rowsplit;
+        c->h_colidx = pat.colidx;
+        c->h_color = pat.color;
+        c->nnz_ff = 0;
+        for (int64_t r = 0; r < d->n_free; ++r) c->nnz_ff += pat.rowsplit[r];
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        // ---- elements in colour order ----
+        std::vector<int64_t> ucount(pat.n_colors + 1, 0), ccount(pat.n_colors + 1, 0);
+        for (int64_t e = 0; e < d->n_elem; ++e) {
+            if (mv.cut_index[e] < 0) ++ucount[pat.color[e] + 1];
+            else ++ccount[pat.color[e] + 1];
+        }
+        for (int k = 0; k < pat.n_colors; ++k) { ucount[k + 1] += ucount[k]; ccount[k + 1] += ccount[k]; }
+        c->ue_color_ptr = ucount;
+        c->ce_color_ptr = ccount;
+        c->n_uncut = ucount[pat.n_colors];
+        std::vector<int32_t> h_conn(4 * std::max<int64_t>(c->n_uncut, 1));
+        std::vector<uint8_t> h_mat(std::max<int64_t>(c->n_uncut, 1));
+        std::vector<uint32_t> h_off(8 * std::max<int64_t>(c->n_uncut, 1));
+        std::vector<int32_t> h_clist(std::max<int64_t>(c->n_cut, 1));
+        {
+            std::vector<int64_t> uf(ucount.begin(), ucount.end() - 1), cf(ccount.begin(), ccount.end() - 1);
+            for (int64_t e = 0; e < d->n_elem; ++e) {
+                int col = pat.color[e];
+                if (mv.cut_index[e] >= 0) { h_clist[cf[col]++] = mv.cut_index[e]; continue; }
+                int64_t p = uf[col]++;
+                for (int k = 0; k < 4; ++k) h_conn[k * c->n_uncut + p] = d->conn[e * 4 + k];
+                if (d->elem_mat[e] < 0 || d->elem_mat[e] >= d->n_mat) throw pgf::Error(PGF_EARG, "element material out of range");
+                h_mat[p] = (uint8_t)d->elem_mat[e];
+                for (int k = 0; k < 8; ++k) {
+                    uint32_t wv;
+                    std::memcpy(&wv, &pat.elem_off[(size_t)e * 32 + k * 4], 4);
+                    h_off[k * c->n_uncut + p] = wv;
+                }
+            }
+        }
+        // ---- materials ----
+        std::vector<double> D(36 * d->n_mat, 0.0);
+        for (int m = 0; m < d->n_mat; ++m) pgf::isotropic_D(d->dim, d->plane_strain != 0, d->mat_E[m], d->mat_nu[m], &D[36 * m]);
+        std::vector<pgfk::CohParams> laws(std::max(1, d->n_coh_law));
+        for (int k = 0; k < d->n_coh_law; ++k) {
+            const double* pp = d->coh_params + 6 * k;
+            laws[k] = pgfk::CohParams{d->coh_type[k], pp[0], pp[1], pp[2], pp[3], pp[4], pp[5]};
+        }
+        // ---- cut elements ----
+        std::vector<int32_t> ce_nodes(16 * std::max<int64_t>(d->n_cut, 1), 0), ce_nt(std::max<int64_t>(d->n_cut, 1), 4);
+        for (int64_t k = 0; k < d->n_cut; ++k) {
+            int64_t e = d->cut_elem[k];
+            ce_nt[k] = 4 + d->cut_n_enrich[k];
+            for (int s = 0; s < ce_nt[k]; ++s) ce_nodes[k * 16 + s] = mv.node(e, s);
+        }
+        const int64_t n_vpt = d->n_cut ? d->vpt_ptr[d->n_cut] : 0;
+        const int64_t n_cpt = d->n_cut ? d->cpt_ptr[d->n_cut] : 0;
+        int64_t n_grad = 0;
+        for (int64_t k = 0; k < d->n_cut; ++k)
+            for (int64_t pt = d->vpt_ptr[k]; pt < d->vpt_ptr[k + 1]; ++pt) {
+                n_grad = std::max<int64_t>(n_grad, d->vpt_grad_off[pt] + (int64_t)ce_nt[k] * d->dim);
+                if (d->vpt_mat[pt] < 0 || d->vpt_mat[pt] >= d->n_mat) throw pgf::Error(PGF_EARG, "point material out of range");
+            }
+        for (int64_t q = 0; q < n_cpt; ++q)
+            if (d->cpt_law[q] < 0 || d->cpt_law[q] >= d->n_coh_law) throw pgf::Error(PGF_EARG, "cohesive law out of range");
+        for (int64_t k = 0; k < d->n_cut; ++k)
+            for (int64_t q = d->cpt_ptr[k]; q < d->cpt_ptr[k + 1]; ++q) {
+                const int nh = d->cpt_nh[q];
+                if (nh < 2 || nh > 4) throw pgf::Error(PGF_EARG, "cohesive point with a bad surface node count");
+                for (int s = 0; s < 2 * nh; ++s)
+                    if (d->cpt_nodes[q * 8 + s] >= ce_nt[k]) throw pgf::Error(PGF_EARG, "cohesive node slot out of range");
+            }
+        c->n_cpt = n_cpt;
+        // ---- free / prescribed slots ----
+        std::vector<int64_t> fslot(std::max<int64_t>(d->n_free, 1)), pslot(std::max<int64_t>(d->n_pres, 1));
+        for (int dd = 0; dd < d->dim; ++dd)
+            for (int64_t n = 0; n < d->n_all_nodes; ++n) {
+                int32_t g = d->dof[(size_t)dd * d->n_all_nodes + n];
+                if (g < d->n_free) fslot[g] = (int64_t)dd * d->n_all_nodes + n;
+                else pslot[g - d->n_free] = (int64_t)dd * d->n_all_nodes + n;
+            }
+        // ---- upload ----
+        std::vector<double> Xh(3 * d->n_all_nodes);
+        std::memcpy(Xh.data(), d->coords, sizeof(double) * 3 * d->n_all_nodes);
+        c->X = pgfd::dupload(Xh.data(), 3 * d->n_all_nodes);
+        c->dof = pgfd::dupload(d->dof, (int64_t)d->dim * d->n_all_nodes);
+        c->U = pgfd::dalloc<double>((int64_t)d->dim * d->n_all_nodes);
+        HIPCHK(hipMemset(c->U, 0, sizeof(double) * d->dim * d->n_all_nodes));
+        c->ue_conn = pgfd::dupload(h_conn.data(), (int64_t)h_conn.size());
+        c->ue_mat = pgfd::dupload(h_mat.data(), (int64_t)h_mat.size());
+        c->ue_off = pgfd::dupload(h_off.data(), (int64_t)h_off.size());
+        c->Dmat = pgfd::dupload(D.data(), (int64_t)D.size());
+        c->laws = pgfd::dupload(laws.data(), (int64_t)laws.size());
+        c->ce_list = pgfd::dupload(h_clist.data(), (int64_t)h_clist.size());
+        c->ce_nodes = pgfd::dupload(ce_nodes.data(), (int64_t)ce_nodes.size());
+        c->ce_nt = pgfd::dupload(ce_nt.data(), (int64_t)ce_nt.size());
+        c->vpt_ptr = pgfd::dupload(d->n_cut ? d->vpt_ptr : nullptr, d->n_cut + 1);
+        c->vpt_w = pgfd::dupload(d->vpt_w, n_vpt);
+        c->vpt_mat = pgfd::dupload(d->vpt_mat, n_vpt);
+        c->vpt_grad_off = pgfd::dupload(d->vpt_grad_off, n_vpt);
+        c->vpt_grad = pgfd::dupload(d->vpt_grad, n_grad);
+        c->cpt_ptr = pgfd::dupload(d->n_cut ? d->cpt_ptr : nullptr, d->n_cut + 1);
+        c->cpt_w = pgfd::dupload(d->cpt_w, n_cpt);
+        c->cpt_N = pgfd::dupload(d->cpt_N, 4 * n_cpt);
+        c->cpt_R = pgfd::dupload(d->cpt_R, 9 * n_cpt);
+        c->cpt_nodes = pgfd::dupload(d->cpt_nodes, 8 * n_cpt);
+        c->cpt_nh = pgfd::dupload(d->cpt_nh, n_cpt);
+        c->cpt_law = pgfd::dupload(d->cpt_law, n_cpt);
+        c->ce_off = pgfd::dupload(pat.cut_off.data(), (int64_t)pat.cut_off.size());
+        c->ce_off_ptr = pgfd::dupload(pat.cut_off_ptr.data(), (int64_t)pat.cut_off_ptr.size());
+        c->dmax_conv = pgfd::dalloc<double>(n_cpt);
+        c->dmax_trial = pgfd::dalloc<double>(n_cpt);
+        c->flags = pgfd::dalloc<uint8_t>(n_cpt);
+        HIPCHK(hipMemset(c->dmax_conv, 0, sizeof(double) * std::max<int64_t>(n_cpt, 1)));
+        HIPCHK(hipMemset(c->dmax_trial, 0, sizeof(double) * std::max<int64_t>(n_cpt, 1)));
+        HIPCHK(hipMemset(c->flags, 0, std::max<int64_t>(n_cpt, 1)));
+        c->free_slot = pgfd::dupload(fslot.data(), (int64_t)fslot.size());
+        c->pres_slot = pgfd::dupload(pslot.data(), (int64_t)pslot.size());
+        std::vector<int32_t> rp32(c->n_rows + 1);
+        for (int64_t r = 0; r <= c->n_rows; ++r) rp32[r] = (int32_t)pat.rowptr[r];
+        c->rowptr = pgfd::dupload(rp32.data(), c->n_rows + 1);
+        c->rowsplit = pgfd::dupload(pat.rowsplit.data(), c->n_rows);
+        c->colidx = pgfd::dupload(pat.colidx.data(), c->nnz);
+        c->K = pgfd::dalloc<double>(c->nnz);
+        c->K32 = pgfd::dalloc<float>(c->nnz);
+        c->P = pgfd::dalloc<double>(c->n_rows);
+        c->Fext_full = pgfd::dalloc<double>(c->n_free);
+        c->upres_full = pgfd::dalloc<double>(c->n_pres);
+        HIPCHK(hipMemset(c->Fext_full, 0, sizeof(double) * std::max<int64_t>(c->n_free, 1)));
+        HIPCHK(hipMemset(c->upres_full, 0, sizeof(double) * std::max<int64_t>(c->n_pres, 1)));
+        const int64_t nf = c->n_free;
+        c->r = pgfd::dalloc<double>(nf); c->x = pgfd::dalloc<double>(nf); c->z = pgfd::dalloc<double>(nf);
+        c->p0 = pgfd::dalloc<double>(nf); c->p1 = pgfd::dalloc<double>(nf); c->q = pgfd::dalloc<double>(nf);
+        c->dinv = pgfd::dalloc<double>(nf); c->guess = pgfd::dalloc<double>(nf); c->res = pgfd::dalloc<double>(nf);
+        c->r32 = pgfd::dalloc<float>(nf); c->x32 = pgfd::dalloc<float>(nf); c->z32 = pgfd::dalloc<float>(nf);
+        c->p032 = pgfd::dalloc<float>(nf); c->p132 = pgfd::dalloc<float>(nf); c->q32 = pgfd::dalloc<float>(nf);
+        c->dinv32 = pgfd::dalloc<float>(nf);
+        c->partial = pgfd::dalloc<double>(3 * 2 * pgfd::RED_BLOCKS);
+        c->scal = pgfd::dalloc<double>(pgfd::S_NSCAL + 8);
+        c->counter = pgfd::dalloc<unsigned>(8);
+        c->done = pgfd::dalloc<int>(1);
+        HIPCHK(hipMemset(c->counter, 0, sizeof(unsigned) * 8));
+        HIPCHK(hipMemset(c->done, 0, sizeof(int)));
+        HIPCHK(hipMemset(c->scal, 0, sizeof(double) * (pgfd::S_NSCAL + 8)));
+        HIPCHK(hipHostMalloc((void**)&c->h_scal, sizeof(double) * (pgfd::S_NSCAL + 8), hipHostMallocDefault));
+        HIPCHK(hipDeviceSynchronize());
+        *out = c;
+        return 0;
+    } catch (const pgf::Error& e) {
+        pgfd::g_last_error = e.what();
+        delete c;
+        return e.code;
+    } catch (const std::exception& e) {
+        pgfd::g_last_error = e.what();
+        delete c;
+        return PGF_EARG;
+    }
+}
+
+PGF_API void pgf_destroy(pgf_ctx* ctx) { delete ctx; }
+
+PGF_API int pgf_csr_info_get(pgf_ctx* c, pgf_csr_info* info) {
+    if (!c || !info) return PGF_EARG;
+    info->n_rows = c->n_rows; info->n_free = c->n_free; info->nnz = c->nnz; info->nnz_ff = c->nnz_ff;
+    info->n_colors = c->n_colors; info->reserved = 0;
+    return 0;
+}
+
+PGF_API int pgf_get_csr(pgf_ctx* c, int64_t* rowptr, int32_t* rowsplit, int32_t* colidx) {
+    if (!c) return PGF_EARG;
+    if (rowptr) std::memcpy(rowptr, c->h_rowptr.data(), sizeof(int64_t) * (c->n_rows + 1));
+    if (rowsplit) std::memcpy(rowsplit, c->h_rowsplit.data(), sizeof(int32_t) * c->n_rows);
+    if (colidx) std::memcpy(colidx, c->h_colidx.data(), sizeof(int32_t) * c->nnz);
+    return 0;
+}
+
+PGF_API int pgf_get_colors(pgf_ctx* c, int32_t* color) {
+    if (!c || !color) return PGF_EARG;
+    std::memcpy(color, c->h_color.data(), sizeof(int32_t) * c->n_elem);
+    return 0;
+}
+
+PGF_API int pgf_set_solution(pgf_ctx* c, const double* u) {
+    if (!c || !u) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        HIPCHK(hipMemcpyAsync(c->U, u, sizeof(double) * c->dim * c->n_all, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+PGF_API int pgf_get_solution(pgf_ctx* c, double* u) {
+    if (!c || !u) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        HIPCHK(hipMemcpyAsync(u, c->U, sizeof(double) * c->dim * c->n_all, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+PGF_API int pgf_set_load(pgf_ctx* c, const double* up, const double* fext) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (up && c->n_pres)
+            HIPCHK(hipMemcpyAsync(c->upres_full, up, sizeof(double) * c->n_pres, hipMemcpyHostToDevice, c->stream));
+        if (fext && c->n_free)
+            HIPCHK(hipMemcpyAsync(c->Fext_full, fext, sizeof(double) * c->n_free, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+PGF_API int pgf_apply_load_fraction(pgf_ctx* c, double frac) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (c->n_pres)
+            hipLaunchKernelGGL(pgfd::k_apply_frac, dim3(pgfd::grid_for(c->n_pres)), dim3(256), 0, c->stream, c->n_pres,
+                               c->pres_slot, c->upres_full, frac, c->U);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+PGF_API int pgf_assemble(pgf_ctx* c, double dt, uint32_t flags) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        pgfd::enqueue_assembly(*c, dt, flags);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+PGF_API int pgf_get_matrix(pgf_ctx* c, double* vals) {
+    if (!c || !vals) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        HIPCHK(hipMemcpy(vals, c->K, sizeof(double) * c->nnz, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+PGF_API int pgf_get_matrix32(pgf_ctx* c, float* vals) {
+    if (!c || !vals) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        HIPCHK(hipMemcpy(vals, c->K32, sizeof(float) * c->nnz, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+PGF_API int pgf_get_internal_force(pgf_ctx* c, double* p) {
+    if (!c || !p) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        HIPCHK(hipMemcpy(p, c->P, sizeof(double) * c->n_rows, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
+PGF_API int pgf_n_cohesive_points(pgf_ctx* c, int64_t* n) {
+    if (!c || !n) return PGF_EARG;
+    *n = c->n_cpt;
+    return 0;
+}
+PGF_API int pgf_isv_commit(pgf_ctx* c) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (c->n_cpt)
+            hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(c->n_cpt)), dim3(256), 0, c->stream, c->n_cpt, c->dmax_trial,
+                               c->dmax_conv);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+PGF_API int pgf_isv_rollback(pgf_ctx* c) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (c->n_cpt)
+            hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(c->n_cpt)), dim3(256), 0, c->stream, c->n_cpt, c->dmax_conv,
+                               c->dmax_trial);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+PGF_API int pgf_get_isv(pgf_ctx* c, double* dt, double* dc, uint8_t* fl) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (c->n_cpt) {
+            if (dt) HIPCHK(hipMemcpy(dt, c->dmax_trial, sizeof(double) * c->n_cpt, hipMemcpyDeviceToHost));
+            if (dc) HIPCHK(hipMemcpy(dc, c->dmax_conv, sizeof(double) * c->n_cpt, hipMemcpyDeviceToHost));
+            if (fl) HIPCHK(hipMemcpy(fl, c->flags, c->n_cpt, hipMemcpyDeviceToHost));
+        }
+        return 0;
+    });
+}
+PGF_API int pgf_set_isv(pgf_ctx* c, const double* dc) {
+    if (!c || !dc) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (c->n_cpt) {
+            HIPCHK(hipMemcpy(c->dmax_conv, dc, sizeof(double) * c->n_cpt, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(c->dmax_trial, dc, sizeof(double) * c->n_cpt, hipMemcpyHostToDevice));
+        }
+        return 0;
+    });
+}
+
+PGF_API int pgf_spmv(pgf_ctx* c, const double* x, double* y) {
+    if (!c || !x || !y) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        HIPCHK(hipMemcpyAsync(c->x, x, sizeof(double) * c->n_free, hipMemcpyHostToDevice, c->stream));
+        pgfd::enqueue_spmv<double>(*c, c->K, c->x, c->q);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(y, c->q, sizeof(double) * c->n_free, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+PGF_API int pgf_pcg(pgf_ctx* c, const double* b, double* x, const double* x0, double rtol, int32_t maxit,
+                    int32_t precision, pgf_solve_info* info) {
+    if (!c || !b || !x) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        const int64_t n = c->n_free;
+        HIPCHK(hipMemcpyAsync(c->res, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        if (precision == 32) {
+            if (x0) {
+                HIPCHK(hipMemcpyAsync(c->guess, x0, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+                hipLaunchKernelGGL(pgfd::k_to_f32, dim3(pgfd::grid_for(n)), dim3(256), 0, c->stream, n, c->guess, c->x32);
+            }
+            pgfd::run_pcg<float>(*c, c->K32, c->res, true, c->x32, c->r32, c->z32, c->p032, c->p132, c->q32, c->dinv32,
+                                 x0 != nullptr, rtol, maxit, info);
+            std::vector<float> tmp(n);
+            HIPCHK(hipMemcpy(tmp.data(), c->x32, sizeof(float) * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) x[i] = tmp[i];
+        } else {
+            if (x0) HIPCHK(hipMemcpyAsync(c->x, x0, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+            pgfd::run_pcg<double>(*c, c->K, c->res, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, x0 != nullptr,
+                                  rtol, maxit, info);
+            HIPCHK(hipMemcpy(x, c->x, sizeof(double) * n, hipMemcpyDeviceToHost));
+        }
+        return 0;
+    });
+}
+
+PGF_API int pgf_set_guess(pgf_ctx* c, const double* dx) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (!dx) { c->have_guess = false; return 0; }
+        HIPCHK(hipMemcpy(c->guess, dx, sizeof(double) * c->n_free, hipMemcpyHostToDevice));
+        c->have_guess = true;
+        return 0;
+    });
+}
+
+// Newton-Raphson for one load step, SolverNonlinearKSP::nonlinearStep
+// (src/SolverNonlinearKSP.cpp:55-159): K and r at the current iterate, linear
+// solve, update, new residual; converged when ||r|| < rel_tol ||r0|| or
+// ||r|| < abs_tol, failed on NaN, iteration cap, or three increases in a row.
+PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, pgf_step_info* info) {
+    if (!c || !o) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        hipEvent_t t0, t1, a0, a1;
+        HIPCHK(hipEventCreate(&t0)); HIPCHK(hipEventCreate(&t1));
+        HIPCHK(hipEventCreate(&a0)); HIPCHK(hipEventCreate(&a1));
+        HIPCHK(hipEventRecord(t0, c->stream));
+        const int64_t nf = c->n_free;
+        double ms_asm = 0.0, ms_sol = 0.0;
+        if (c->n_pres)
+            hipLaunchKernelGGL(pgfd::k_apply_frac, dim3(pgfd::grid_for(c->n_pres)), dim3(256), 0, c->stream, c->n_pres,
+                               c->pres_slot, c->upres_full, frac, c->U);
+        auto residual = [&](double& rn) {
+            HIPCHK(hipEventRecord(a0, c->stream));
+            pgfd::enqueue_assembly(*c, o->dt, PGF_ASM_FUNC);
+            hipLaunchKernelGGL(pgfd::k_residual, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->Fext_full, frac,
+                               c->P, c->r, c->partial, c->counter + 3, c->scal + pgfd::S_NSCAL);
+            HIPCHK(hipMemcpyAsync(c->h_scal + pgfd::S_NSCAL, c->scal + pgfd::S_NSCAL, sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipEventRecord(a1, c->stream));
+            HIPCHK(hipEventSynchronize(a1));
+            float ms;
+            HIPCHK(hipEventElapsedTime(&ms, a0, a1));
+            ms_asm += ms;
+            rn = std::sqrt(c->h_scal[pgfd::S_NSCAL]);
+        };
+        double res0, res;
+        residual(res0);
+        res = res0;
+        int it = 0, inc = 0, pcg_its = 0;
+        bool conv = res0 < o->abs_tol || res0 == 0.0, fail = false;
+        while (!conv && !fail) {
+            const bool fp32 = o->precision == 32;
+            HIPCHK(hipEventRecord(a0, c->stream));
+            pgfd::enqueue_assembly(*c, o->dt, PGF_ASM_JAC | (fp32 ? PGF_ASM_FP32 : 0u));
+            HIPCHK(hipEventRecord(a1, c->stream));
+            HIPCHK(hipEventSynchronize(a1));
+            float ms;
+            HIPCHK(hipEventElapsedTime(&ms, a0, a1));
+            ms_asm += ms;
+            pgf_solve_info si{};
+            const bool g = o->use_guess && c->have_guess && it == 0;
+            if (fp32) {
+                if (g) hipLaunchKernelGGL(pgfd::k_to_f32, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x32);
+                pgfd::run_pcg<float>(*c, c->K32, c->r, true, c->x32, c->r32, c->z32, c->p032, c->p132, c->q32, c->dinv32, g,
+                                     o->lin_rtol, o->lin_maxit, &si);
+                hipLaunchKernelGGL((pgfd::k_update_U<float>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
+                                   c->free_slot, c->x32, c->U);
+            } else {
+                if (g) hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x);
+                pgfd::run_pcg<double>(*c, c->K, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g, o->lin_rtol,
+                                      o->lin_maxit, &si);
+                hipLaunchKernelGGL((pgfd::k_update_U<double>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
+                                   c->free_slot, c->x, c->U);
+            }
+            ms_sol += si.ms;
+            pcg_its += si.iterations;
+            ++it;
+            double prev = res;
+            residual(res);
+            if (res < o->rel_tol * res0 || res < o->abs_tol) { conv = true; break; }
+            if (!std::isfinite(res)) { fail = true; break; }
+            if (it >= o->max_iter) { fail = true; break; }
+            inc = res > prev ? inc + 1 : 0;
+            if (inc >= 3) { fail = true; break; }
+        }
+        HIPCHK(hipEventRecord(t1, c->stream));
+        HIPCHK(hipEventSynchronize(t1));
+        float mt;
+        HIPCHK(hipEventElapsedTime(&mt, t0, t1));
+        (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); (void)hipEventDestroy(a0); (void)hipEventDestroy(a1);
+        if (info) {
+            info->newton_iterations = it;
+            info->converged = conv ? 1 : 0;
+            info->pcg_iterations = pcg_its;
+            info->res0 = res0;
+            info->res = res;
+            info->ms_total = mt;
+            info->ms_assemble = ms_asm;
+            info->ms_solve = ms_sol;
+        }
+        return conv ? 0 : PGF_ENOTCONVERGED;
+    });
+}
+
+PGF_API int pgf_time_assemble(pgf_ctx* c, uint32_t flags, int32_t reps, double* ms_per_call) {
+    if (!c || reps <= 0 || !ms_per_call) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+        pgfd::enqueue_assembly(*c, 1.0, flags);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; ++i) pgfd::enqueue_assembly(*c, 1.0, flags);
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_call = ms / reps;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        return 0;
+    });
+}
+
+PGF_API int pgf_time_spmv(pgf_ctx* c, int32_t precision, int32_t reps, double* ms_per_call) {
+    if (!c || reps <= 0 || !ms_per_call) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+        if (precision == 32) { pgfd::fill_async<float>(*c, c->x32, c->n_free, 1.0f); pgfd::enqueue_spmv<float>(*c, c->K32, c->x32, c->q32); }
+        else { pgfd::fill_async<double>(*c, c->x, c->n_free, 1.0); pgfd::enqueue_spmv<double>(*c, c->K, c->x, c->q); }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; ++i) {
+            if (precision == 32) pgfd::enqueue_spmv<float>(*c, c->K32, c->x32, c->q32);
+            else pgfd::enqueue_spmv<double>(*c, c->K, c->x, c->q);
+        }
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_call = ms / reps;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Device-resident context of one mesh (what pgf_ctx points to).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pgf_kernels.hpp"
+
+namespace pgfd {
+
+struct Ctx {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int dim = 0, kind = 0, n_colors = 0;
+    double tet_scale = 1.0;
+    int64_t n_nodes = 0, n_all = 0, n_elem = 0, n_uncut = 0, n_cut = 0, n_free = 0, n_pres = 0, n_rows = 0,
+            nnz = 0, nnz_ff = 0, n_cpt = 0, n_mat = 0;
+    // host copies of the pattern (exported for parity checks)
+    std::vector<int64_t> h_rowptr;
+    std::vector<int32_t> h_rowsplit, h_colidx, h_color;
+    std::vector<int64_t> ue_color_ptr, ce_color_ptr;
+    // mesh
+    double* X = nullptr;
+    int32_t* dof = nullptr;
+    double* U = nullptr;
+    int32_t* ue_conn = nullptr;
+    uint8_t* ue_mat = nullptr;
+    uint32_t* ue_off = nullptr;
+    double* Dmat = nullptr;
+    int32_t *ce_list = nullptr, *ce_nodes = nullptr, *ce_nt = nullptr;
+    int64_t *vpt_ptr = nullptr, *vpt_grad_off = nullptr, *cpt_ptr = nullptr, *ce_off_ptr = nullptr;
+    double *vpt_w = nullptr, *vpt_grad = nullptr, *cpt_w = nullptr, *cpt_N = nullptr, *cpt_R = nullptr;
+    int32_t *vpt_mat = nullptr, *cpt_law = nullptr;
+    uint8_t *cpt_nodes = nullptr, *cpt_nh = nullptr, *ce_off = nullptr;
+    pgfk::CohParams* laws = nullptr;
+    double *dmax_conv = nullptr, *dmax_trial = nullptr;
+    uint8_t* flags = nullptr;
+    // free / prescribed dof -> nodal field slot (d*n_all + node)
+    int64_t *free_slot = nullptr, *pres_slot = nullptr;
+    // matrix
+    int32_t *rowptr = nullptr, *rowsplit = nullptr, *colidx = nullptr;
+    double* K = nullptr;
+    float* K32 = nullptr;
+    double* P = nullptr;
+    double* Fext_full = nullptr;   // [n_free]
+    double* upres_full = nullptr;  // [n_pres]
+    // solver work (FP64 and FP32)
+    double *r = nullptr, *x = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr,
+           *guess = nullptr, *res = nullptr;
+    float *r32 = nullptr, *x32 = nullptr, *z32 = nullptr, *p032 = nullptr, *p132 = nullptr, *q32 = nullptr,
+          *dinv32 = nullptr;
+    double* partial = nullptr;     // block partial sums
+    double* scal = nullptr;        // device scalars of the PCG recurrence
+    unsigned* counter = nullptr;   // last-block counters
+    int* done = nullptr;
+    double* h_scal = nullptr;      // pinned mirror
+    bool have_guess = false;
+
+    pgfk::AsmArgs asm_args() const;
+    ~Ctx();
+};
+
+template <typename T>
+void fill_async(Ctx& c, T* p, int64_t n, T v);
+void enqueue_assembly(Ctx& c, double dt, uint32_t flags);
+
+}  // namespace pgfd

@@ -1,0 +1,143 @@
+// Device kernels of the PIGFEM MI355X path (gfx950, wave64).
+//
+//  * k_uncut<DIM,...>   one thread per uncut parent element of one colour:
+//                       geometry from the nodal coordinates, per-QP B^T D B and
+//                       B^T sigma, read-modify-write into the CSR rows (no
+//                       atomics: elements of one colour share no node).
+//  * k_cut<DIM,...>     one wavefront per cut (IGFEM) element: integration
+//                       tables staged in LDS, element matrix accumulated in LDS,
+//                       cohesive points (traction-separation law + delta_max
+//                       update) folded in, then scattered like k_uncut.
+//  * k_spmv_p / k_cg_update   fused Jacobi-PCG iteration (two kernels per
+//                       iteration, deterministic last-block reductions).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace pgfk {
+
+constexpr int WAVE = 64;
+constexpr int MAX_NT = 12;          // element slots of a cut element
+constexpr int MAX_M = MAX_NT * 3;   // element dofs
+
+struct CohParams {
+    int type;
+    double sc, dc, df, beta, visc, tfloor;
+};
+
+// Arguments shared by the assembly kernels (passed by value).
+struct AsmArgs {
+    int64_t n_all, n_free, n_rows, n_uncut;
+    const double* X;        // [3][n_all]
+    const int32_t* dof;     // [dim][n_all]
+    const double* U;        // [dim][n_all]
+    // uncut elements in colour order
+    const int32_t* ue_conn;  // [4][n_uncut]
+    const uint8_t* ue_mat;   // [n_uncut]
+    const uint32_t* ue_off;  // [8][n_uncut]
+    const double* Dmat;      // [n_mat][36]
+    double tet_scale;
+    // cut elements
+    const int32_t* ce_list;      // cut indices in colour order
+    const int32_t* ce_nodes;     // [n_cut][16]
+    const int32_t* ce_nt;        // [n_cut]
+    const int64_t* vpt_ptr;
+    const double* vpt_w;
+    const int32_t* vpt_mat;
+    const int64_t* vpt_grad_off;
+    const double* vpt_grad;
+    const int64_t* cpt_ptr;
+    const double* cpt_w;
+    const double* cpt_N;
+    const double* cpt_R;
+    const uint8_t* cpt_nodes;
+    const uint8_t* cpt_nh;
+    const int32_t* cpt_law;
+    const CohParams* laws;
+    const uint8_t* ce_off;
+    const int64_t* ce_off_ptr;
+    const double* dmax_conv;
+    double* dmax_trial;
+    uint8_t* flags;
+    double dt;
+    // outputs
+    const int32_t* rowptr;
+    double* K;
+    float* K32;
+    double* P;            // [n_rows]
+    int all_rows;
+};
+
+// ---- Ortiz-Pandolfi / bilinear traction-separation law ---------------------
+// src/Material_OPCohesive.cpp:38-319 (active exponential law with linear
+// unloading and viscous term; penetration branch disabled as in the reference,
+// :115), delta_eff from src/Material_OPCohesiveNU.cpp:45-63, damage zone from
+// :224-236.  Bilinear: the commented law of :175-199/256-271 with delta_c and
+// delta_f as parameters.  The irreversibility test and the delta_max update are
+// FP64 in every precision mode.
+__device__ inline void coh_law(const CohParams& L, int dim, double* delta, double dmax, double dt,
+                               double* trac, double* Dc, double& dmax_new, uint8_t& flag) {
+    const double b2 = L.beta * L.beta;
+    double norm = 0.0;
+    for (int i = 0; i < dim; ++i) norm += delta[i] * delta[i];
+    if (norm < 1e-40)
+        for (int i = 0; i < dim; ++i) delta[i] = 1e-20;
+    double deff = delta[dim - 1] * delta[dim - 1];
+    for (int i = 0; i < dim - 1; ++i) deff += b2 * (delta[i] * delta[i]);
+    deff = sqrt(deff);
+    double T, dT;
+    const bool loading = deff >= dmax || deff < 0.0;
+    if (L.type == 0) {
+        if (loading) {
+            T = L.sc * (deff / L.dc) * exp(1.0 - deff / L.dc);
+            dT = (L.sc / L.dc) * (1.0 - deff / L.dc) * exp(1.0 - deff / L.dc);
+            if (L.visc != 0.0) {
+                T += L.visc / L.sc * (deff - dmax) / dt / L.dc;
+                dT += L.visc / L.sc / dt / L.dc;
+            }
+            dmax_new = deff;
+        } else {
+            double tmax = L.sc * (dmax / L.dc) * exp(1.0 - dmax / L.dc);
+            T = tmax * (deff / dmax);
+            dT = tmax / dmax;
+            dmax_new = dmax;
+        }
+    } else {
+        auto curve = [&](double x, double& slope) {
+            if (x <= L.dc) { slope = L.sc / L.dc; return L.sc / L.dc * x; }
+            if (x <= L.df) { slope = -L.sc / (L.df - L.dc); return -L.sc / (L.df - L.dc) * (x - L.dc) + L.sc; }
+            slope = 0.0;
+            return 0.0;
+        };
+        if (loading) {
+            T = curve(deff, dT);
+            if (T < L.tfloor && deff > L.dc) { T = L.tfloor; dT = 0.0; }
+            dmax_new = deff;
+        } else {
+            double s;
+            double tmax = curve(dmax, s);
+            T = tmax * (deff / dmax);
+            dT = tmax / dmax;
+            dmax_new = dmax;
+        }
+    }
+    double br[3], dd[3];
+    for (int i = 0; i < dim; ++i) br[i] = b2 * delta[i];
+    br[dim - 1] += (1.0 - b2) * delta[dim - 1];
+    for (int i = 0; i < dim; ++i) trac[i] = (T / deff) * br[i];
+    dd[dim - 1] = delta[dim - 1] / deff;
+    for (int i = 0; i < dim - 1; ++i) dd[i] = b2 * delta[i] / deff;
+    for (int i = 0; i < dim; ++i)
+        for (int k = 0; k < dim; ++k) {
+            double kr = (i == k ? 1.0 : 0.0);
+            double kn = (i == dim - 1 && k == dim - 1) ? 1.0 : 0.0;
+            Dc[i * dim + k] = br[i] * (dT * deff - T) * (dd[k] / (deff * deff)) + (T / deff) * (b2 * kr + (1.0 - b2) * kn);
+        }
+    double nz = deff / L.dc;
+    uint8_t zone = nz <= 0 ? 0 : (nz <= 1 ? 1 : (nz <= 5.0 ? 2 : 3));
+    flag = (uint8_t)((loading ? 1 : 0) | (zone << 1) | (dmax_new > L.dc ? 8 : 0) |
+                     ((L.type == 1 && dmax_new >= L.df) ? 16 : 0));
+}
+
+}  // namespace pgfk
