@@ -62,6 +62,7 @@ extern "C" {
 #define PGF_ASM_JAC 2u       /* stiffness K */
 #define PGF_ASM_FP32 4u      /* K in FP32 (mixed-precision path); P_int stays FP64 */
 #define PGF_ASM_ALL_ROWS 8u  /* also assemble prescribed rows (K_pf, K_pp); default free rows only */
+#define PGF_ASM_COLORED 16u  /* element-coloured read-modify-write scheme instead of the row gather */
 
 typedef struct pgf_ctx pgf_ctx;
 
@@ -195,6 +196,8 @@ PGF_API int pgf_newton_step(pgf_ctx* ctx, double frac, const pgf_newton_opts* op
 /* ---- measurement ---- */
 /* time `reps` calls of pgf_assemble(flags) with HIP events on the context stream */
 PGF_API int pgf_time_assemble(pgf_ctx* ctx, uint32_t flags, int32_t reps, double* ms_per_call);
+/* per-pass device time of the row-gather assembly (out[0] call, out[1] cut-element pass, out[2] gather pass) */
+PGF_API int pgf_time_assemble_split(pgf_ctx* ctx, uint32_t flags, int32_t reps, double* out);
 /* time `reps` K_ff SpMVs (FP64 or FP32) on device-resident vectors */
 PGF_API int pgf_time_spmv(pgf_ctx* ctx, int32_t precision, int32_t reps, double* ms_per_call);
 

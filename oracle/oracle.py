@@ -47,6 +47,7 @@ def lib():
                                       C.c_int, C.c_void_p]
         L.orc_detect_nodes.argtypes = [C.c_int64, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_law.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_void_p]
+        L.orc_sample_range.argtypes = [C.c_int64, C.c_int64]
         L.orc_edge_root.restype = C.c_double
         L.orc_edge_root.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -61,8 +62,10 @@ class Oracle:
     """Reference-structured CPU model built from a HostProblem's geometry and
     cut topology.  `bc` is a list of (node, dof, value)."""
 
-    def __init__(self, prob, bcs):
+    def __init__(self, prob, bcs, sample=None):
+        """sample=(e0, e1) builds and assembles only elements [e0, e1) (timing)."""
         L = lib()
+        L.orc_sample_range(*(sample if sample else (-1, -1)))
         self.prob = prob
         i = prob.info
         self.dim = i.dim
@@ -82,6 +85,7 @@ class Oracle:
         self.h = L.orc_create(self.dim, int(prob.plane_strain), 1.0 if prob.kind != "tet4" else prob_tet_scale(prob),
                               i.n_nodes, i.n_all_nodes, _p(X), i.n_elem, _p(conn), _p(em), len(E), _p(E), _p(nu),
                               laws.shape[0], _p(laws), i.n_cut, _p(cut), len(bcs), _p(bn), _p(bd), _p(bv))
+        L.orc_sample_range(-1, -1)
         sz = np.zeros(4, np.int64)
         L.orc_sizes(self.h, _p(sz))
         self.n_free, self.n_pres, self.n_isv, self.n_qp = [int(v) for v in sz]

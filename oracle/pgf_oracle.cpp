@@ -200,6 +200,7 @@ struct Problem {
     int dim = 2, kind = 0, plane_strain = 1;
     double tet_scale = 1.0;
     int64_t n_nodes = 0, n_all = 0, n_elem = 0;
+    int64_t e_begin = 0, e_end = 0;  // elements whose tables are built and assembled
     Mat X;                                   // [n_all][3]
     std::vector<std::vector<int>> elem_nodes;  // all slots per element
     std::vector<std::vector<QP>> qps;        // per element
@@ -270,7 +271,7 @@ void build_tables(Problem& P, const int32_t* cut, int64_t n_cut) {
     P.qps.assign(P.n_elem, {});
     int64_t isv = 0;
     // elem_nodes holds the 4 regular nodes on entry; enrichment slots are appended
-    for (int64_t e = 0; e < P.n_elem; ++e) {
+    for (int64_t e = P.e_begin; e < P.e_end; ++e) {
         std::vector<int>& nodes = P.elem_nodes[e];
         if (cidx[e] < 0) {
             Mat PX(4);
@@ -608,7 +609,7 @@ Assembled assemble(const Problem& Pb, const Vec& U, double dt, const Vec& dconv,
     A.P.assign(Pb.n_free + Pb.n_pres, 0.0);
     dtrial = dconv;
     flags.assign(Pb.n_isv, 0);
-    for (int64_t e = 0; e < Pb.n_elem; ++e) {
+    for (int64_t e = Pb.e_begin; e < Pb.e_end; ++e) {
         Mat Ke;
         Vec Pe;
         element(Pb, e, U, dt, dconv, dtrial, flags, bvar, Ke, Pe);
@@ -685,6 +686,16 @@ struct OrcHandle {
 
 extern "C" {
 
+static int64_t g_e_begin = -1, g_e_end = -1;
+
+// Restrict the next orc_create to elements [e0, e1) (tables and assembly):
+// used only to time a bounded sample of a large mesh.  -1, -1 restores the
+// whole mesh.
+ORC_API void orc_sample_range(int64_t e0, int64_t e1) {
+    g_e_begin = e0;
+    g_e_end = e1;
+}
+
 // cut: n_cut records of 128 int32 (pgfh_get_cut layout); laws: [n_law][7] = type, sc, dc, df, beta, visc, tfloor
 ORC_API void* orc_create(int dim, int plane_strain, double tet_scale, int64_t n_nodes, int64_t n_all,
                          const double* coords, int64_t n_elem, const int32_t* conn, const int32_t* elem_mat, int n_mat,
@@ -710,6 +721,8 @@ ORC_API void* orc_create(int dim, int plane_strain, double tet_scale, int64_t n_
         P.laws.push_back(Law{(int)l[0], l[1], l[2], l[3], l[4], l[5], l[6]});
     }
     for (int64_t e = 0; e < n_elem; ++e) P.elem_nodes[e] = {conn[e * 4], conn[e * 4 + 1], conn[e * 4 + 2], conn[e * 4 + 3]};
+    P.e_begin = g_e_begin < 0 ? 0 : g_e_begin;
+    P.e_end = g_e_end < 0 ? n_elem : std::min<int64_t>(g_e_end, n_elem);
     build_tables(P, cut, n_cut);
     number_dofs(P, bc_node, bc_dof, bc_val, n_bc);
     h->U.assign((size_t)dim * n_all, 0.0);

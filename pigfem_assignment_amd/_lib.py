@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libpgfem.so")
 
 PGF_QUAD4, PGF_TET4 = 0, 1
 PGF_LAW_OP_EXPONENTIAL, PGF_LAW_BILINEAR = 0, 1
-PGF_ASM_FUNC, PGF_ASM_JAC, PGF_ASM_FP32, PGF_ASM_ALL_ROWS = 1, 2, 4, 8
+PGF_ASM_FUNC, PGF_ASM_JAC, PGF_ASM_FP32, PGF_ASM_ALL_ROWS, PGF_ASM_COLORED = 1, 2, 4, 8, 16
 SET_NODE, SET_LEFT, SET_RIGHT, SET_BOTTOM, SET_TOP, SET_FRONT, SET_BACK = -1, 0, 1, 2, 3, 4, 5
 
 dp = C.POINTER(C.c_double)
@@ -97,7 +97,7 @@ EXPORTS = [
     "pgf_get_colors", "pgf_set_solution", "pgf_get_solution", "pgf_set_load", "pgf_apply_load_fraction",
     "pgf_assemble", "pgf_get_matrix", "pgf_get_matrix32", "pgf_get_internal_force", "pgf_n_cohesive_points",
     "pgf_isv_commit", "pgf_isv_rollback", "pgf_get_isv", "pgf_set_isv", "pgf_spmv", "pgf_pcg", "pgf_set_guess",
-    "pgf_newton_step", "pgf_time_assemble", "pgf_time_spmv",
+    "pgf_newton_step", "pgf_time_assemble", "pgf_time_assemble_split", "pgf_time_spmv",
     "pgfh_problem_create", "pgfh_problem_destroy", "pgfh_last_error", "pgfh_problem_info", "pgfh_mesh_desc",
     "pgfh_get_coords", "pgfh_get_conn", "pgfh_get_elem_mat", "pgfh_get_node_detect", "pgfh_get_dofs",
     "pgfh_get_enrich_edges", "pgfh_get_cut", "pgfh_pattern_info", "pgfh_get_pattern", "pgfh_solve_load_steps",
@@ -125,6 +125,7 @@ def lib():
     L.pgf_pcg.argtypes = [C.c_void_p, dp, dp, dp, C.c_double, C.c_int32, C.c_int32, C.POINTER(SolveInfo)]
     L.pgf_newton_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(NewtonOpts), C.POINTER(StepInfo)]
     L.pgf_time_assemble.argtypes = [C.c_void_p, C.c_uint32, C.c_int32, dp]
+    L.pgf_time_assemble_split.argtypes = [C.c_void_p, C.c_uint32, C.c_int32, dp]
     L.pgf_time_spmv.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
     L.pgfh_problem_create.argtypes = [C.POINTER(HostConfig), C.POINTER(C.c_void_p)]
     L.pgfh_solve_load_steps.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(LoadOpts), STEP_CB, C.c_void_p,

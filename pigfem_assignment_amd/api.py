@@ -253,6 +253,12 @@ class DeviceContext:
         self._chk(self.lib.pgf_time_assemble(self.h, flags, reps, C.byref(ms)))
         return ms.value
 
+    def time_assemble_split(self, flags, reps=10):
+        """(ms per call, ms cut-element pass, ms row-gather pass)"""
+        out = np.zeros(3)
+        self._chk(self.lib.pgf_time_assemble_split(self.h, flags, reps, L.ptr(out, C.c_double)))
+        return tuple(out)
+
     def time_spmv(self, precision=64, reps=20):
         ms = C.c_double()
         self._chk(self.lib.pgf_time_spmv(self.h, precision, reps, C.byref(ms)))

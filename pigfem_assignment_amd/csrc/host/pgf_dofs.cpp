@@ -1,9 +1,26 @@
 #include "pgf_dofs.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <sstream>
 
 namespace pgf {
+
+namespace {
+// phase timing of the set-up, printed when PGF_TIMING is set
+struct PhaseTimer {
+    bool on = std::getenv("PGF_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char* what) {
+        if (!on) return;
+        auto n = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[pattern] %-24s %.3f s\n", what, std::chrono::duration<double>(n - t).count());
+        t = n;
+    }
+};
+}  // namespace
 
 void DofMap::build(const Mesh& mesh, const std::vector<Dirichlet>& bcs) {
     dim = mesh.dim();
@@ -75,6 +92,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
     const int dim = mesh.dim;
     const int64_t na = mesh.n_all, ne = mesh.n_elem;
     const int32_t* conn = mesh.conn;
+    PhaseTimer tm;
     // node -> element incidence over every element slot (regular and enrichment)
     std::vector<int64_t> nptr(na + 1, 0);
     for (int64_t e = 0; e < ne; ++e)
@@ -91,6 +109,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
             for (int s = 0; s < mesh.cut_n_enrich[c]; ++s)
                 nel[fill[mesh.n_nodes + mesh.cut_enrich[c * 16 + s]]++] = mesh.cut_elem[c];
     }
+    tm.mark("incidence");
     // node neighbour lists (sorted all-node ids)
     std::vector<int64_t> nbptr(na + 1, 0);
     std::vector<std::vector<int32_t>> nbtmp(na);
@@ -130,6 +149,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
         nfree_row[n] = fr;
     }
     if (overflow) throw Error(PGF_ERR_LIMIT, "row wider than 255 columns");
+    tm.mark("neighbour lists");
     nbtmp.clear();
     nbtmp.shrink_to_fit();
     // row layout: every row of node n = free dofs of its neighbours (node order),
@@ -159,6 +179,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
             }
         rowsplit[r] = nfree_row[n];
     }
+    tm.mark("csr rows");
     // element scatter offsets
     auto lookup = [&](int32_t a, int32_t b, uint8_t& of, uint8_t& op) {
         const int32_t* beg = &nb[nbptr[a]];
@@ -190,6 +211,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
                 lookup(mesh.node(e, i), mesh.node(e, j), cut_off[cut_off_ptr[c] + (i * nt + j) * 2],
                        cut_off[cut_off_ptr[c] + (i * nt + j) * 2 + 1]);
     }
+    tm.mark("scatter offsets");
     // every scatter slot must land on its own column inside its own row: this
     // check keeps the device scatter in bounds by construction
     bool bad = false;
@@ -222,6 +244,31 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
             bad = true;
     }
     if (bad) throw Error(PGF_ERR_STATE, "scatter offsets do not match the CSR pattern");
+    tm.mark("offset validation");
+    // row-gather plan from the node -> element incidence (element order per node)
+    inc_ptr.assign(nptr.begin(), nptr.end());
+    inc_elem.assign(nel.begin(), nel.end());
+    inc_slot.assign(nel.size(), 0);
+    inc_off.assign(nel.size(), 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t n = 0; n < na; ++n)
+        for (int64_t k = nptr[n]; k < nptr[n + 1]; ++k) {
+            const int64_t e = nel[k];
+            const int nt = mesh.n_total(e);
+            int la = -1;
+            for (int s = 0; s < nt; ++s)
+                if (mesh.node(e, s) == n) { la = s; break; }
+            inc_slot[k] = (uint8_t)la;
+            if (mesh.cut_index[e] < 0) {
+                uint64_t w = 0;
+                for (int b = 0; b < 4; ++b) {
+                    w |= (uint64_t)elem_off[(size_t)e * 32 + (la * 4 + b) * 2] << (16 * b);
+                    w |= (uint64_t)elem_off[(size_t)e * 32 + (la * 4 + b) * 2 + 1] << (16 * b + 8);
+                }
+                inc_off[k] = w;
+            }
+        }
+    tm.mark("gather plan");
     // greedy colouring in element order over the regular nodes (elements that
     // share an enrichment node always share that edge's regular nodes)
     color.assign(ne, 0);
@@ -236,6 +283,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
         for (int k = 0; k < 4; ++k) mask[conn[e * 4 + k]] |= 1ull << c;
         n_colors = std::max(n_colors, c + 1);
     }
+    tm.mark("colouring");
 }
 
 }  // namespace pgf

@@ -67,6 +67,13 @@ struct Pattern {
     // greedy colouring: no two elements of one colour share a node
     int n_colors = 0;
     std::vector<int32_t> color;      // per element
+    // row-gather plan: for every node its incident elements (element order),
+    // the node's slot in each, and for uncut elements the packed (free, pres)
+    // column offsets of the element's 4 nodes inside the node's rows
+    std::vector<int64_t> inc_ptr;    // [n_all+1]
+    std::vector<int32_t> inc_elem;   // [n_inc]
+    std::vector<uint8_t> inc_slot;   // [n_inc]
+    std::vector<uint64_t> inc_off;   // [n_inc] uncut elements only
     void build(const MeshView& mesh, const int32_t* dof, int64_t n_free, int64_t n_pres);
 };
 

@@ -8,7 +8,17 @@
 // (SmallStrainInternalForceFast, :364-408), cohesive opening / stiffness /
 // force (cohesiveDeltaFast, cohesiveKFast, cohesiveInternalForceFast,
 // :465-531).  The scatter is Assembler::set_matrix / set_vec
-// (src/Assembler.cpp:811-850) made atomic-free by element colouring.
+// (src/Assembler.cpp:811-850).
+//
+// Two launch schemes, both free of atomics:
+//  * gather (default): one thread per node owns that node's CSR rows, zeroes
+//    them and accumulates the contributions of its incident elements in
+//    element order (the reference's summation order).  Uncut elements are
+//    evaluated on the fly; cut elements are evaluated once per element by one
+//    wavefront into a dense scratch block first.  Every K entry reaches HBM
+//    about once; the read-modify-writes stay in L2.
+//  * coloured: one thread per uncut element / one wavefront per cut element,
+//    colour by colour, read-modify-write straight into the CSR rows.
 #include "pgf_kernels.hpp"
 #include "pgf_device.hpp"
 
@@ -81,39 +91,26 @@ __device__ inline double force_entry(const double* g, int d, const double* s) {
     return g[2] * s[2] + g[1] * s[3] + g[0] * s[4];
 }
 
-// ---------------------------------------------------------------------------
-// Uncut elements: one thread per element.
-template <int DIM, bool DO_K, bool DO_F, typename KT>
-__global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t end) {
-    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= end) return;
-    constexpr int S = DIM == 2 ? 3 : 6;
-    constexpr int NQ = DIM == 2 ? 4 : 1;  // linear tets: one merged point
-    int nd[4];
-    double X[4][DIM];
-    for (int k = 0; k < 4; ++k) {
-        nd[k] = a.ue_conn[k * a.n_uncut + p];
-        for (int i = 0; i < DIM; ++i) X[k][i] = a.X[i * a.n_all + nd[k]];
-    }
-    const int mat = a.ue_mat[p];
-    double D[S * S];
-    for (int k = 0; k < S * S; ++k) D[k] = a.Dmat[mat * 36 + k];
-    double g[NQ][4 * DIM], w[NQ];
+// Geometry of an uncut parent element: QUAD4 2x2 Gauss points (w = J*1) or
+// the linear tet's single merged point (the five-point rule's weights
+// -0.8 + 4*0.45 applied to the common B), Elem::ShapeFunctions at each.
+template <int DIM>
+__device__ inline void uncut_geometry(const double (*X)[DIM], double tet_scale, double (*g)[4 * DIM], double* w) {
     if (DIM == 2) {
         const double r0 = 0.57735026918962576450914878050196;
         for (int q = 0; q < 4; ++q) {
-            double r = (q % 2 == 0) ? -r0 : r0, s = (q / 2 == 0) ? -r0 : r0;
-            double dr[4][2] = {{-0.25 * (1.0 - s), -0.25 * (1.0 - r)},
-                               {0.25 * (1.0 - s), -0.25 * (1.0 + r)},
-                               {0.25 * (1.0 + s), 0.25 * (1.0 + r)},
-                               {-0.25 * (1.0 + s), 0.25 * (1.0 - r)}};
+            const double r = (q % 2 == 0) ? -r0 : r0, s = (q / 2 == 0) ? -r0 : r0;
+            const double dr[4][2] = {{-0.25 * (1.0 - s), -0.25 * (1.0 - r)},
+                                     {0.25 * (1.0 - s), -0.25 * (1.0 + r)},
+                                     {0.25 * (1.0 + s), 0.25 * (1.0 + r)},
+                                     {-0.25 * (1.0 + s), 0.25 * (1.0 - r)}};
             double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
             for (int n = 0; n < 4; ++n) {
                 J00 += dr[n][0] * X[n][0]; J01 += dr[n][0] * X[n][1];
                 J10 += dr[n][1] * X[n][0]; J11 += dr[n][1] * X[n][1];
             }
-            double det = J00 * J11 - J01 * J10;
-            double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
+            const double det = J00 * J11 - J01 * J10;
+            const double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
             for (int n = 0; n < 4; ++n) {
                 g[q][n * 2 + 0] = i00 * dr[n][0] + i01 * dr[n][1];
                 g[q][n * 2 + 1] = i10 * dr[n][0] + i11 * dr[n][1];
@@ -121,15 +118,15 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
             w[q] = det * 1.0;
         }
     } else {
-        // dN/dr constant: rows of the tet map are x1-x0, x2-x0, x3-x0
         double J[3][3];
         for (int i = 0; i < 3; ++i) {
             J[0][i] = X[1][i] - X[0][i];
             J[1][i] = X[2][i] - X[0][i];
             J[2][i] = X[3][i] - X[0][i];
         }
-        double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
         double inv[3][3];
         inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
         inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
@@ -143,11 +140,37 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
         const double dr[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
         for (int n = 0; n < 4; ++n)
             for (int i = 0; i < 3; ++i) g[0][n * 3 + i] = inv[i][0] * dr[n][0] + inv[i][1] * dr[n][1] + inv[i][2] * dr[n][2];
-        // five-point rule, weights -0.8, 4 x 0.45 (sum 1), all points share B
         double ws = det * -0.8;
         for (int q = 0; q < 4; ++q) ws += det * 0.45;
-        w[0] = ws * a.tet_scale;
+        w[0] = ws * tet_scale;
     }
+}
+
+template <typename KT>
+__device__ inline void kadd(const AsmArgs& a, int64_t pos, double v) {
+    if (sizeof(KT) == 8) a.K[pos] += v;
+    else a.K32[pos] += (float)v;
+}
+
+// ---------------------------------------------------------------------------
+// Coloured scheme, uncut elements: one thread per element.
+template <int DIM, bool DO_K, bool DO_F, typename KT>
+__global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t end) {
+    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    constexpr int S = DIM == 2 ? 3 : 6;
+    constexpr int NQ = DIM == 2 ? 4 : 1;
+    int nd[4];
+    double X[4][DIM];
+    for (int k = 0; k < 4; ++k) {
+        nd[k] = a.ue_conn[k * a.n_uncut + p];
+        for (int i = 0; i < DIM; ++i) X[k][i] = a.X[i * a.n_all + nd[k]];
+    }
+    const int mat = a.ue_mat[p];
+    double D[S * S];
+    for (int k = 0; k < S * S; ++k) D[k] = a.Dmat[mat * 36 + k];
+    double g[NQ][4 * DIM], w[NQ];
+    uncut_geometry<DIM>(X, a.tet_scale, g, w);
     int row_dof[4][DIM];
     for (int n = 0; n < 4; ++n)
         for (int d = 0; d < DIM; ++d) row_dof[n][d] = a.dof[d * a.n_all + nd[n]];
@@ -166,7 +189,7 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
         }
         for (int n = 0; n < 4; ++n)
             for (int d = 0; d < DIM; ++d) {
-                int r = row_dof[n][d];
+                const int r = row_dof[n][d];
                 if (r < a.n_free || a.all_rows) a.P[r] += Pe[n * DIM + d];
             }
     }
@@ -195,8 +218,7 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
                     for (int db = 0; db < DIM; ++db) {
                         const int c = row_dof[bn][db];
                         const int64_t pos = c < a.n_free ? base + of + cf++ : base + op + cp++;
-                        if (sizeof(KT) == 8) a.K[pos] += Kr[bn * DIM + db];
-                        else a.K32[pos] += (float)Kr[bn * DIM + db];
+                        kadd<KT>(a, pos, Kr[bn * DIM + db]);
                     }
                 }
             }
@@ -204,21 +226,15 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
 }
 
 // ---------------------------------------------------------------------------
-// Cut elements: one wavefront per element, element matrix in LDS.
-template <int DIM, bool DO_K, bool DO_F, typename KT>
-__global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
-    const int64_t li = begin + blockIdx.x;
-    if (li >= end) return;
-    const int c = a.ce_list[li];
+// Cut element computed by one wavefront into LDS: K_e (M x M) and P_e (M).
+// Also evaluates the cohesive law at the element's cohesive points and writes
+// their trial delta_max and damage flags.
+template <int DIM, bool DO_K, bool DO_F>
+__device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K, double* s_P, int* s_node, int* s_dof) {
     const int tid = threadIdx.x;
     constexpr int S = DIM == 2 ? 3 : 6;
-    const int nt = a.ce_nt[c];
     const int M = nt * DIM;
-    __shared__ int s_node[MAX_NT];
-    __shared__ int s_dof[MAX_M];
     __shared__ double s_u[MAX_M];
-    __shared__ double s_K[MAX_M * MAX_M];
-    __shared__ double s_P[MAX_M];
     __shared__ double s_g[MAX_NT * 3];
     __shared__ double s_DB[6 * MAX_M];
     __shared__ double s_sig[6];
@@ -229,12 +245,12 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
     __syncthreads();
     for (int i = tid; i < M; i += WAVE) {
         s_dof[i] = a.dof[(i % DIM) * a.n_all + s_node[i / DIM]];
-        s_u[i] = DO_F ? a.U[(i % DIM) * a.n_all + s_node[i / DIM]] : 0.0;
+        s_u[i] = a.U[(i % DIM) * a.n_all + s_node[i / DIM]];
         s_P[i] = 0.0;
     }
-    for (int i = tid; i < M * M; i += WAVE) s_K[i] = 0.0;
+    if (DO_K)
+        for (int i = tid; i < M * M; i += WAVE) s_K[i] = 0.0;
     __syncthreads();
-    // volumetric points
     for (int64_t pt = a.vpt_ptr[c]; pt < a.vpt_ptr[c + 1]; ++pt) {
         const double* gp = a.vpt_grad + a.vpt_grad_off[pt];
         for (int i = tid; i < M; i += WAVE) s_g[i] = gp[i];
@@ -247,10 +263,8 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
             strain_of<DIM>(nt, s_g, s_u, eps);
             stress_of<DIM>(s_D, eps, s_sig);
         }
-        if (DO_K) {
-            // parts for every element row (a, da): s_DB[i*S + k]
+        if (DO_K)
             for (int i = tid; i < M; i += WAVE) k_parts<DIM>(&s_g[(i / DIM) * DIM], i % DIM, s_D, &s_DB[i * S]);
-        }
         __syncthreads();
         if (DO_K)
             for (int e = tid; e < M * M; e += WAVE) {
@@ -261,7 +275,6 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
             for (int i = tid; i < M; i += WAVE) s_P[i] += w * force_entry<DIM>(&s_g[(i / DIM) * DIM], i % DIM, s_sig);
         __syncthreads();
     }
-    // cohesive points
     for (int64_t q = a.cpt_ptr[c]; q < a.cpt_ptr[c + 1]; ++q) {
         if (tid == 0) {
             const int nh = a.cpt_nh[q];
@@ -279,7 +292,6 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
             coh_law(a.laws[a.cpt_law[q]], DIM, delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
             a.dmax_trial[q] = dnew;
             a.flags[q] = fl;
-            // R^T D R: (R^T D) then * R
             double RtD[9];
             for (int i = 0; i < DIM; ++i)
                 for (int j = 0; j < DIM; ++j) {
@@ -322,7 +334,19 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
             }
         __syncthreads();
     }
-    // scatter
+}
+
+// Coloured scheme, cut elements: compute then scatter into the CSR rows.
+template <int DIM, bool DO_K, bool DO_F, typename KT>
+__global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
+    const int64_t li = begin + blockIdx.x;
+    if (li >= end) return;
+    const int c = a.ce_list[li];
+    const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
+    __shared__ double s_K[MAX_M * MAX_M];
+    __shared__ double s_P[MAX_M];
+    __shared__ int s_node[MAX_NT], s_dof[MAX_M];
+    cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
     const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
     if (DO_F)
         for (int i = tid; i < M; i += WAVE) {
@@ -336,20 +360,145 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
             if (r >= a.n_free && !a.all_rows) continue;
             const int an = i / DIM, bn = j / DIM, db = j % DIM;
             const int of = off[(an * nt + bn) * 2], op = off[(an * nt + bn) * 2 + 1];
-            // rank of db among node bn's free / prescribed dofs
             int cf = 0, cp = 0;
             for (int k = 0; k < db; ++k) { if (s_dof[bn * DIM + k] < a.n_free) ++cf; else ++cp; }
             const int cdof = s_dof[j];
             const int64_t pos = cdof < a.n_free ? a.rowptr[r] + of + cf : a.rowptr[r] + op + cp;
-            if (sizeof(KT) == 8) a.K[pos] += s_K[e];
-            else a.K32[pos] += (float)s_K[e];
+            kadd<KT>(a, pos, s_K[e]);
         }
+}
+
+// Gather scheme, pass 1: every cut element's K_e / P_e into the dense scratch.
+template <int DIM, bool DO_K, bool DO_F>
+__global__ __launch_bounds__(WAVE) void k_cut_elem(AsmArgs a) {
+    const int c = blockIdx.x;
+    const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
+    __shared__ double s_K[MAX_M * MAX_M];
+    __shared__ double s_P[MAX_M];
+    __shared__ int s_node[MAX_NT], s_dof[MAX_M];
+    cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
+    if (DO_K) {
+        double* ke = a.ke + a.ke_ptr[c];
+        for (int e = tid; e < M * M; e += WAVE) ke[e] = s_K[e];
+    }
+    if (DO_F) {
+        double* pe = a.pe + a.pe_ptr[c];
+        for (int i = tid; i < M; i += WAVE) pe[i] = s_P[i];
+    }
+}
+
+// Gather scheme, pass 2: one thread per node owns the node's rows.
+template <int DIM, bool DO_K, bool DO_F, typename KT>
+__global__ __launch_bounds__(256) void k_gather(AsmArgs a) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.n_all) return;
+    constexpr int S = DIM == 2 ? 3 : 6;
+    constexpr int NQ = DIM == 2 ? 4 : 1;
+    int rr[DIM];
+    bool own[DIM], any = false;
+    for (int d = 0; d < DIM; ++d) {
+        rr[d] = a.dof[d * a.n_all + n];
+        own[d] = rr[d] < a.n_free || a.all_rows;
+        any |= own[d];
+    }
+    if (!any) return;
+    if (DO_K)
+        for (int d = 0; d < DIM; ++d)
+            if (own[d])
+                for (int32_t j = a.rowptr[rr[d]]; j < a.rowptr[rr[d] + 1]; ++j) {
+                    if (sizeof(KT) == 8) a.K[j] = 0.0;
+                    else a.K32[j] = 0.0f;
+                }
+    double Pa[DIM];
+    for (int d = 0; d < DIM; ++d) Pa[d] = 0.0;
+    for (int64_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
+        const int e = a.inc_elem[k];
+        const int la = a.inc_slot[k];
+        const int c = a.cut_index[e];
+        if (c < 0) {
+            int nd[4];
+            double X[4][DIM];
+            for (int q = 0; q < 4; ++q) {
+                nd[q] = a.el_conn[(int64_t)q * a.n_elem + e];
+                for (int i = 0; i < DIM; ++i) X[q][i] = a.X[i * a.n_all + nd[q]];
+            }
+            const int mat = a.el_mat[e];
+            double D[S * S];
+            for (int q = 0; q < S * S; ++q) D[q] = a.Dmat[mat * 36 + q];
+            double g[NQ][4 * DIM], w[NQ];
+            uncut_geometry<DIM>(X, a.tet_scale, g, w);
+            if (DO_F) {
+                double u[4 * DIM];
+                for (int q = 0; q < 4; ++q)
+                    for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
+                for (int q = 0; q < NQ; ++q) {
+                    double eps[S], sig[S];
+                    strain_of<DIM>(4, g[q], u, eps);
+                    stress_of<DIM>(D, eps, sig);
+                    for (int d = 0; d < DIM; ++d) Pa[d] += w[q] * force_entry<DIM>(&g[q][la * DIM], d, sig);
+                }
+            }
+            if (DO_K) {
+                const uint64_t off = a.inc_off[k];
+                int cd[4][DIM];
+                for (int q = 0; q < 4; ++q)
+                    for (int d = 0; d < DIM; ++d) cd[q][d] = a.dof[d * a.n_all + nd[q]];
+                for (int da = 0; da < DIM; ++da) {
+                    if (!own[da]) continue;
+                    double Kr[4 * DIM];
+                    for (int j = 0; j < 4 * DIM; ++j) Kr[j] = 0.0;
+                    for (int q = 0; q < NQ; ++q) {
+                        double parts[S];
+                        k_parts<DIM>(&g[q][la * DIM], da, D, parts);
+                        for (int bn = 0; bn < 4; ++bn)
+                            for (int db = 0; db < DIM; ++db)
+                                Kr[bn * DIM + db] += w[q] * k_entry<DIM>(parts, &g[q][bn * DIM], db);
+                    }
+                    const int64_t base = a.rowptr[rr[da]];
+                    for (int bn = 0; bn < 4; ++bn) {
+                        const int of = (int)((off >> (16 * bn)) & 0xff), op = (int)((off >> (16 * bn + 8)) & 0xff);
+                        int cf = 0, cp = 0;
+                        for (int db = 0; db < DIM; ++db) {
+                            const int64_t pos = cd[bn][db] < a.n_free ? base + of + cf++ : base + op + cp++;
+                            kadd<KT>(a, pos, Kr[bn * DIM + db]);
+                        }
+                    }
+                }
+            }
+        } else {
+            const int nt = a.ce_nt[c], M = nt * DIM;
+            if (DO_F)
+                for (int d = 0; d < DIM; ++d) Pa[d] += a.pe[a.pe_ptr[c] + la * DIM + d];
+            if (DO_K) {
+                const double* ke = a.ke + a.ke_ptr[c];
+                const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
+                for (int da = 0; da < DIM; ++da) {
+                    if (!own[da]) continue;
+                    const int64_t base = a.rowptr[rr[da]];
+                    const double* krow = ke + (int64_t)(la * DIM + da) * M;
+                    for (int bn = 0; bn < nt; ++bn) {
+                        const int nb = a.ce_nodes[c * 16 + bn];
+                        const int of = off[(la * nt + bn) * 2], op = off[(la * nt + bn) * 2 + 1];
+                        int cf = 0, cp = 0;
+                        for (int db = 0; db < DIM; ++db) {
+                            const int cdof = a.dof[db * a.n_all + nb];
+                            const int64_t pos = cdof < a.n_free ? base + of + cf++ : base + op + cp++;
+                            kadd<KT>(a, pos, krow[bn * DIM + db]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (DO_F)
+        for (int d = 0; d < DIM; ++d)
+            if (own[d]) a.P[rr[d]] = Pa[d];
 }
 
 }  // namespace pgfk
 
 // ---------------------------------------------------------------------------
-// Launch one full assembly (all colours) on the context stream.
+// Launch one full assembly on the context stream.
 namespace pgfd {
 
 template <int DIM, bool DO_K, bool DO_F, typename KT>
@@ -367,27 +516,45 @@ static void launch_colors(const Ctx& c, const pgfk::AsmArgs& a) {
     }
 }
 
+template <int DIM, bool DO_K, bool DO_F, typename KT>
+static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
+    if (c.timing_events) (void)hipEventRecord(c.timing_events[0], c.stream);
+    if (c.n_cut > 0)
+        hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)c.n_cut), dim3(pgfk::WAVE), 0, c.stream, a);
+    if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
+    const unsigned nb = (unsigned)((c.n_all + 255) / 256);
+    hipLaunchKernelGGL((pgfk::k_gather<DIM, DO_K, DO_F, KT>), dim3(nb), dim3(256), 0, c.stream, a);
+    if (c.timing_events) (void)hipEventRecord(c.timing_events[2], c.stream);
+}
+
 template <int DIM>
-static void launch_dim(const Ctx& c, const pgfk::AsmArgs& a, bool jac, bool func, bool fp32) {
-    if (jac && func && !fp32) launch_colors<DIM, true, true, double>(c, a);
-    else if (jac && !fp32) launch_colors<DIM, true, false, double>(c, a);
-    else if (jac && fp32) launch_colors<DIM, true, false, float>(c, a);
-    if (func && (fp32 || !jac)) launch_colors<DIM, false, true, double>(c, a);
+static void launch_dim(Ctx& c, const pgfk::AsmArgs& a, bool jac, bool func, bool fp32, bool colored) {
+    if (colored) {
+        if (jac) {
+            if (fp32) fill_async(c, c.K32, c.nnz, 0.0f);
+            else fill_async(c, c.K, c.nnz, 0.0);
+        }
+        if (func) fill_async(c, c.P, c.n_rows, 0.0);
+        if (jac && func && !fp32) launch_colors<DIM, true, true, double>(c, a);
+        else if (jac && !fp32) launch_colors<DIM, true, false, double>(c, a);
+        else if (jac && fp32) launch_colors<DIM, true, false, float>(c, a);
+        if (func && (fp32 || !jac)) launch_colors<DIM, false, true, double>(c, a);
+        return;
+    }
+    if (jac && func && !fp32) launch_gather<DIM, true, true, double>(c, a);
+    else if (jac && !fp32) launch_gather<DIM, true, false, double>(c, a);
+    else if (jac && fp32) launch_gather<DIM, true, false, float>(c, a);
+    if (func && (fp32 || !jac)) launch_gather<DIM, false, true, double>(c, a);
 }
 
 void enqueue_assembly(Ctx& c, double dt, uint32_t flags) {
     const bool jac = flags & 2u, func = flags & 1u, fp32 = flags & 4u;
-    const bool all_rows = flags & 8u;
+    const bool all_rows = flags & 8u, colored = flags & 16u;
     pgfk::AsmArgs a = c.asm_args();
     a.dt = dt;
     a.all_rows = all_rows ? 1 : 0;
-    if (jac) {
-        if (fp32) fill_async(c, c.K32, c.nnz, 0.0f);
-        else fill_async(c, c.K, c.nnz, 0.0);
-    }
-    if (func) fill_async(c, c.P, c.n_rows, 0.0);
-    if (c.dim == 2) launch_dim<2>(c, a, jac, func, fp32);
-    else launch_dim<3>(c, a, jac, func, fp32);
+    if (c.dim == 2) launch_dim<2>(c, a, jac, func, fp32, colored);
+    else launch_dim<3>(c, a, jac, func, fp32, colored);
 }
 
 }  // namespace pgfd

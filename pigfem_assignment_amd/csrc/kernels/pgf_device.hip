@@ -60,6 +60,9 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.cpt_law = cpt_law; a.laws = laws; a.ce_off = ce_off; a.ce_off_ptr = ce_off_ptr;
     a.dmax_conv = dmax_conv; a.dmax_trial = dmax_trial; a.flags = flags;
     a.rowptr = rowptr; a.K = K; a.K32 = K32; a.P = P;
+    a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index;
+    a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_off = inc_off;
+    a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     return a;
 }
 
@@ -68,7 +71,8 @@ Ctx::~Ctx() {
                     ce_off_ptr, vpt_w, vpt_grad, cpt_w, cpt_N, cpt_R, vpt_mat, cpt_law, cpt_nodes, cpt_nh, ce_off, laws,
                     dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, r32, x32, z32, p032, p132, q32, dinv32, partial,
-                    scal, counter, done};
+                    scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
+                    inc_off, ke, pe};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -549,6 +553,35 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         c->cpt_law = pgfd::dupload(d->cpt_law, n_cpt);
         c->ce_off = pgfd::dupload(pat.cut_off.data(), (int64_t)pat.cut_off.size());
         c->ce_off_ptr = pgfd::dupload(pat.cut_off_ptr.data(), (int64_t)pat.cut_off_ptr.size());
+        // gather scheme: element-order connectivity / material, incidence plan,
+        // dense scratch for the cut elements' K_e and P_e
+        {
+            std::vector<int32_t> ec(4 * std::max<int64_t>(d->n_elem, 1));
+            std::vector<uint8_t> em(std::max<int64_t>(d->n_elem, 1));
+            for (int64_t e = 0; e < d->n_elem; ++e) {
+                for (int k = 0; k < 4; ++k) ec[k * d->n_elem + e] = d->conn[e * 4 + k];
+                em[e] = (uint8_t)d->elem_mat[e];
+            }
+            c->el_conn = pgfd::dupload(ec.data(), (int64_t)ec.size());
+            c->el_mat = pgfd::dupload(em.data(), (int64_t)em.size());
+            c->cut_index = pgfd::dupload(mv.cut_index.data(), d->n_elem);
+            c->n_inc = (int64_t)pat.inc_elem.size();
+            c->inc_ptr = pgfd::dupload(pat.inc_ptr.data(), (int64_t)pat.inc_ptr.size());
+            c->inc_elem = pgfd::dupload(pat.inc_elem.data(), c->n_inc);
+            c->inc_slot = pgfd::dupload(pat.inc_slot.data(), c->n_inc);
+            c->inc_off = pgfd::dupload(pat.inc_off.data(), c->n_inc);
+            std::vector<int64_t> kp(d->n_cut + 1, 0), pp(d->n_cut + 1, 0);
+            for (int64_t k = 0; k < d->n_cut; ++k) {
+                const int64_t m = (int64_t)ce_nt[k] * d->dim;
+                kp[k + 1] = kp[k] + m * m;
+                pp[k + 1] = pp[k] + m;
+            }
+            c->n_ke = kp[d->n_cut];
+            c->ke_ptr = pgfd::dupload(kp.data(), d->n_cut + 1);
+            c->pe_ptr = pgfd::dupload(pp.data(), d->n_cut + 1);
+            c->ke = pgfd::dalloc<double>(kp[d->n_cut]);
+            c->pe = pgfd::dalloc<double>(pp[d->n_cut]);
+        }
         c->dmax_conv = pgfd::dalloc<double>(n_cpt);
         c->dmax_trial = pgfd::dalloc<double>(n_cpt);
         c->flags = pgfd::dalloc<uint8_t>(n_cpt);
@@ -893,6 +926,42 @@ PGF_API int pgf_time_assemble(pgf_ctx* c, uint32_t flags, int32_t reps, double* 
         float ms;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         *ms_per_call = ms / reps;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        return 0;
+    });
+}
+
+// Per-pass device time of the gather assembly, averaged over `reps` calls:
+// out[0] whole call, out[1] cut-element pass, out[2] row-gather pass.
+PGF_API int pgf_time_assemble_split(pgf_ctx* c, uint32_t flags, int32_t reps, double* out) {
+    if (!c || reps <= 0 || !out || (flags & 16u)) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        hipEvent_t ev[3], e0, e1;
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+        double cut = 0.0, gat = 0.0;
+        pgfd::enqueue_assembly(*c, 1.0, flags);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; ++i) {
+            c->timing_events = ev;
+            pgfd::enqueue_assembly(*c, 1.0, flags);
+            c->timing_events = nullptr;
+            HIPCHK(hipEventSynchronize(ev[2]));
+            float a, b;
+            HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+            cut += a;
+            gat += b;
+        }
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float tot;
+        HIPCHK(hipEventElapsedTime(&tot, e0, e1));
+        out[0] = tot / reps;
+        out[1] = cut / reps;
+        out[2] = gat / reps;
+        for (auto& e : ev) (void)hipEventDestroy(e);
         (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
         return 0;
     });

@@ -37,6 +37,13 @@ struct Ctx {
     pgfk::CohParams* laws = nullptr;
     double *dmax_conv = nullptr, *dmax_trial = nullptr;
     uint8_t* flags = nullptr;
+    // gather scheme
+    int32_t *el_conn = nullptr, *cut_index = nullptr, *inc_elem = nullptr;
+    uint8_t *el_mat = nullptr, *inc_slot = nullptr;
+    int64_t *inc_ptr = nullptr, *ke_ptr = nullptr, *pe_ptr = nullptr;
+    uint64_t* inc_off = nullptr;
+    double *ke = nullptr, *pe = nullptr;
+    int64_t n_inc = 0, n_ke = 0;
     // free / prescribed dof -> nodal field slot (d*n_all + node)
     int64_t *free_slot = nullptr, *pres_slot = nullptr;
     // matrix
@@ -57,6 +64,7 @@ struct Ctx {
     int* done = nullptr;
     double* h_scal = nullptr;      // pinned mirror
     bool have_guess = false;
+    hipEvent_t* timing_events = nullptr;  // when set: cut pass start, gather start, gather end
 
     pgfk::AsmArgs asm_args() const;
     ~Ctx();

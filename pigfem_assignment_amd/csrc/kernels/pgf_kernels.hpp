@@ -61,6 +61,19 @@ struct AsmArgs {
     double* dmax_trial;
     uint8_t* flags;
     double dt;
+    // gather scheme: element-order arrays and the node -> element incidence
+    int64_t n_elem;
+    const int32_t* el_conn;    // [4][n_elem]
+    const uint8_t* el_mat;     // [n_elem]
+    const int32_t* cut_index;  // [n_elem], -1 if uncut
+    const int64_t* inc_ptr;    // [n_all+1]
+    const int32_t* inc_elem;
+    const uint8_t* inc_slot;
+    const uint64_t* inc_off;
+    double* ke;                // cut-element K_e scratch
+    const int64_t* ke_ptr;
+    double* pe;                // cut-element P_e scratch
+    const int64_t* pe_ptr;
     // outputs
     const int32_t* rowptr;
     double* K;
