@@ -227,113 +227,133 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
 
 // ---------------------------------------------------------------------------
 // Cut element computed by one wavefront into LDS: K_e (M x M) and P_e (M).
-// Also evaluates the cohesive law at the element's cohesive points and writes
-// their trial delta_max and damage flags.
+// Every integration point is staged in LDS first (gradients, J*W, the point's
+// D, its stress; per cohesive point the rotated tangent R^T D_c R, the rotated
+// traction and the surface shape functions), then each lane owns a set of
+// K_e entries and sums them over the points in point order (volumetric points,
+// then cohesive points): three barriers per element.  The cohesive law runs
+// once per cohesive point (lane q), which also writes that point's trial
+// delta_max and damage flags.
 template <int DIM, bool DO_K, bool DO_F>
 __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K, double* s_P, int* s_node, int* s_dof) {
     const int tid = threadIdx.x;
     constexpr int S = DIM == 2 ? 3 : 6;
+    constexpr int MM = MAX_NT * DIM;
+    constexpr int NV = max_vpt(DIM);
     const int M = nt * DIM;
-    __shared__ double s_u[MAX_M];
-    __shared__ double s_g[MAX_NT * 3];
-    __shared__ double s_DB[6 * MAX_M];
-    __shared__ double s_sig[6];
-    __shared__ double s_D[36];
-    __shared__ double s_RDR[9], s_tR[3], s_N[4], s_w;
-    __shared__ int s_map[8], s_nh;
+    __shared__ double s_u[MM];
+    __shared__ double s_g[NV * MM];
+    __shared__ double s_w[NV];
+    __shared__ double s_D[NV * S * S];
+    __shared__ double s_sig[NV * S];
+    __shared__ double s_cRDR[MAX_CPT * DIM * DIM], s_ctR[MAX_CPT * DIM], s_cN[MAX_CPT * 4], s_cw[MAX_CPT];
+    __shared__ int s_cmap[MAX_CPT * 8], s_cnh[MAX_CPT];
+    const int64_t v0 = a.vpt_ptr[c];
+    const int nv = (int)(a.vpt_ptr[c + 1] - v0);
+    const int64_t c0 = a.cpt_ptr[c];
+    const int nc = (int)(a.cpt_ptr[c + 1] - c0);
     if (tid < nt) s_node[tid] = a.ce_nodes[c * 16 + tid];
     __syncthreads();
     for (int i = tid; i < M; i += WAVE) {
         s_dof[i] = a.dof[(i % DIM) * a.n_all + s_node[i / DIM]];
         s_u[i] = a.U[(i % DIM) * a.n_all + s_node[i / DIM]];
-        s_P[i] = 0.0;
     }
-    if (DO_K)
-        for (int i = tid; i < M * M; i += WAVE) s_K[i] = 0.0;
+    for (int k = tid; k < nv * M; k += WAVE) {
+        const int q = k / M;
+        s_g[q * MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
+    }
+    for (int k = tid; k < nv * S * S; k += WAVE) {
+        const int q = k / (S * S), l = k - q * S * S;
+        s_D[k] = a.Dmat[a.vpt_mat[v0 + q] * 36 + l];
+    }
+    if (tid < nv) s_w[tid] = a.vpt_w[v0 + tid];
     __syncthreads();
-    for (int64_t pt = a.vpt_ptr[c]; pt < a.vpt_ptr[c + 1]; ++pt) {
-        const double* gp = a.vpt_grad + a.vpt_grad_off[pt];
-        for (int i = tid; i < M; i += WAVE) s_g[i] = gp[i];
-        const int mat = a.vpt_mat[pt];
-        if (tid < S * S) s_D[tid] = a.Dmat[mat * 36 + tid];
-        const double w = a.vpt_w[pt];
-        __syncthreads();
-        if (DO_F && tid == 0) {
-            double eps[S];
-            strain_of<DIM>(nt, s_g, s_u, eps);
-            stress_of<DIM>(s_D, eps, s_sig);
-        }
-        if (DO_K)
-            for (int i = tid; i < M; i += WAVE) k_parts<DIM>(&s_g[(i / DIM) * DIM], i % DIM, s_D, &s_DB[i * S]);
-        __syncthreads();
-        if (DO_K)
-            for (int e = tid; e < M * M; e += WAVE) {
-                const int i = e / M, j = e % M;
-                s_K[e] += w * k_entry<DIM>(&s_DB[i * S], &s_g[(j / DIM) * DIM], j % DIM);
-            }
-        if (DO_F)
-            for (int i = tid; i < M; i += WAVE) s_P[i] += w * force_entry<DIM>(&s_g[(i / DIM) * DIM], i % DIM, s_sig);
-        __syncthreads();
+    if (DO_F && tid < nv) {
+        double eps[S];
+        strain_of<DIM>(nt, &s_g[tid * MM], s_u, eps);
+        stress_of<DIM>(&s_D[tid * S * S], eps, &s_sig[tid * S]);
     }
-    for (int64_t q = a.cpt_ptr[c]; q < a.cpt_ptr[c + 1]; ++q) {
-        if (tid == 0) {
-            const int nh = a.cpt_nh[q];
-            double N[4], R[9], dl[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
-            int map[8];
-            for (int k = 0; k < 4; ++k) N[k] = a.cpt_N[q * 4 + k];
-            for (int k = 0; k < 9; ++k) R[k] = a.cpt_R[q * 9 + k];
-            for (int k = 0; k < 2 * nh; ++k) map[k] = a.cpt_nodes[q * 8 + k];
-            for (int d = 0; d < DIM; ++d)
-                for (int n = 0; n < nh; ++n) dl[d] += N[n] * (s_u[map[nh + n] * DIM + d] - s_u[map[n] * DIM + d]);
-            for (int i = 0; i < DIM; ++i)
-                for (int j = 0; j < DIM; ++j) delta[i] += R[i * DIM + j] * dl[j];
-            double trac[3], Dc[9], dnew;
-            uint8_t fl;
-            coh_law(a.laws[a.cpt_law[q]], DIM, delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
-            a.dmax_trial[q] = dnew;
-            a.flags[q] = fl;
-            double RtD[9];
-            for (int i = 0; i < DIM; ++i)
-                for (int j = 0; j < DIM; ++j) {
-                    double s = 0.0;
-                    for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * Dc[k * DIM + j];
-                    RtD[i * DIM + j] = s;
-                }
-            for (int i = 0; i < DIM; ++i)
-                for (int j = 0; j < DIM; ++j) {
-                    double s = 0.0;
-                    for (int k = 0; k < DIM; ++k) s += RtD[i * DIM + k] * R[k * DIM + j];
-                    s_RDR[i * DIM + j] = s;
-                }
-            for (int i = 0; i < DIM; ++i) {
+    if (tid < nc) {
+        const int64_t q = c0 + tid;
+        const int nh = a.cpt_nh[q];
+        double N[4], R[9], dl[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
+        int map[8];
+        for (int k = 0; k < 4; ++k) N[k] = a.cpt_N[q * 4 + k];
+        for (int k = 0; k < 9; ++k) R[k] = a.cpt_R[q * 9 + k];
+        for (int k = 0; k < 8; ++k) map[k] = k < 2 * nh ? (int)a.cpt_nodes[q * 8 + k] : -1;
+        for (int d = 0; d < DIM; ++d)
+            for (int n = 0; n < nh; ++n) dl[d] += N[n] * (s_u[map[nh + n] * DIM + d] - s_u[map[n] * DIM + d]);
+        for (int i = 0; i < DIM; ++i)
+            for (int j = 0; j < DIM; ++j) delta[i] += R[i * DIM + j] * dl[j];
+        double trac[3], Dc[9], dnew;
+        uint8_t fl;
+        coh_law(a.laws[a.cpt_law[q]], DIM, delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
+        a.dmax_trial[q] = dnew;
+        a.flags[q] = fl;
+        double RtD[9];
+        for (int i = 0; i < DIM; ++i)
+            for (int j = 0; j < DIM; ++j) {
                 double s = 0.0;
-                for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * trac[k];
-                s_tR[i] = s;
+                for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * Dc[k * DIM + j];
+                RtD[i * DIM + j] = s;
             }
-            for (int k = 0; k < 4; ++k) s_N[k] = N[k];
-            for (int k = 0; k < 8; ++k) s_map[k] = k < 2 * nh ? map[k] : 0;
-            s_nh = nh;
-            s_w = a.cpt_w[q];
+        for (int i = 0; i < DIM; ++i)
+            for (int j = 0; j < DIM; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < DIM; ++k) s += RtD[i * DIM + k] * R[k * DIM + j];
+                s_cRDR[tid * DIM * DIM + i * DIM + j] = s;
+            }
+        for (int i = 0; i < DIM; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * trac[k];
+            s_ctR[tid * DIM + i] = s;
         }
-        __syncthreads();
-        const int nh = s_nh, mc = 2 * nh * DIM;
-        const double w = s_w;
-        if (DO_K)
-            for (int e = tid; e < mc * mc; e += WAVE) {
-                const int i = e / mc, j = e % mc;
-                const int n1 = i / DIM, d1 = i % DIM, n2 = j / DIM, d2 = j % DIM;
-                const double v = s_N[n1 % nh] * s_N[n2 % nh] * s_RDR[d1 * DIM + d2];
-                const double sgn = ((n1 < nh) == (n2 < nh)) ? 1.0 : -1.0;
-                s_K[(s_map[n1] * DIM + d1) * M + s_map[n2] * DIM + d2] += w * (sgn * v);
-            }
-        if (DO_F)
-            for (int i = tid; i < mc; i += WAVE) {
-                const int n = i / DIM, d = i % DIM;
-                const double v = s_N[n % nh] * s_tR[d];
-                s_P[s_map[n] * DIM + d] += w * (n < nh ? -v : v);
-            }
-        __syncthreads();
+        for (int k = 0; k < 4; ++k) s_cN[tid * 4 + k] = N[k];
+        for (int k = 0; k < 8; ++k) s_cmap[tid * 8 + k] = map[k];
+        s_cnh[tid] = nh;
+        s_cw[tid] = a.cpt_w[q];
     }
+    __syncthreads();
+    if (DO_K)
+        for (int e = tid; e < M * M; e += WAVE) {
+            const int i = e / M, j = e - (e / M) * M;
+            const int an = i / DIM, da = i % DIM, bn = j / DIM, db = j % DIM;
+            double acc = 0.0;
+            for (int q = 0; q < nv; ++q) {
+                double parts[S];
+                k_parts<DIM>(&s_g[q * MM + an * DIM], da, &s_D[q * S * S], parts);
+                acc += s_w[q] * k_entry<DIM>(parts, &s_g[q * MM + bn * DIM], db);
+            }
+            for (int q = 0; q < nc; ++q) {
+                const int nh = s_cnh[q];
+                for (int k1 = 0; k1 < 2 * nh; ++k1) {
+                    if (s_cmap[q * 8 + k1] != an) continue;
+                    for (int k2 = 0; k2 < 2 * nh; ++k2) {
+                        if (s_cmap[q * 8 + k2] != bn) continue;
+                        const double v = s_cN[q * 4 + k1 % nh] * s_cN[q * 4 + k2 % nh] * s_cRDR[q * DIM * DIM + da * DIM + db];
+                        const double sgn = ((k1 < nh) == (k2 < nh)) ? 1.0 : -1.0;
+                        acc += s_cw[q] * (sgn * v);
+                    }
+                }
+            }
+            s_K[e] = acc;
+        }
+    if (DO_F)
+        for (int i = tid; i < M; i += WAVE) {
+            const int an = i / DIM, d = i % DIM;
+            double acc = 0.0;
+            for (int q = 0; q < nv; ++q) acc += s_w[q] * force_entry<DIM>(&s_g[q * MM + an * DIM], d, &s_sig[q * S]);
+            for (int q = 0; q < nc; ++q) {
+                const int nh = s_cnh[q];
+                for (int k = 0; k < 2 * nh; ++k)
+                    if (s_cmap[q * 8 + k] == an) {
+                        const double v = s_cN[q * 4 + k % nh] * s_ctR[q * DIM + d];
+                        acc += s_cw[q] * (k < nh ? -v : v);
+                    }
+            }
+            s_P[i] = acc;
+        }
+    __syncthreads();
 }
 
 // Coloured scheme, cut elements: compute then scatter into the CSR rows.
@@ -341,11 +361,12 @@ template <int DIM, bool DO_K, bool DO_F, typename KT>
 __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
     const int64_t li = begin + blockIdx.x;
     if (li >= end) return;
+    constexpr int MM = MAX_NT * DIM;
     const int c = a.ce_list[li];
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
-    __shared__ double s_K[MAX_M * MAX_M];
-    __shared__ double s_P[MAX_M];
-    __shared__ int s_node[MAX_NT], s_dof[MAX_M];
+    __shared__ double s_K[MM * MM];
+    __shared__ double s_P[MM];
+    __shared__ int s_node[MAX_NT], s_dof[MM];
     cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
     const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
     if (DO_F)
@@ -371,11 +392,12 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
 // Gather scheme, pass 1: every cut element's K_e / P_e into the dense scratch.
 template <int DIM, bool DO_K, bool DO_F>
 __global__ __launch_bounds__(WAVE) void k_cut_elem(AsmArgs a) {
+    constexpr int MM = MAX_NT * DIM;
     const int c = blockIdx.x;
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
-    __shared__ double s_K[MAX_M * MAX_M];
-    __shared__ double s_P[MAX_M];
-    __shared__ int s_node[MAX_NT], s_dof[MAX_M];
+    __shared__ double s_K[MM * MM];
+    __shared__ double s_P[MM];
+    __shared__ int s_node[MAX_NT], s_dof[MM];
     cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
     if (DO_K) {
         double* ke = a.ke + a.ke_ptr[c];
@@ -387,95 +409,98 @@ __global__ __launch_bounds__(WAVE) void k_cut_elem(AsmArgs a) {
     }
 }
 
-// Gather scheme, pass 2: one thread per node owns the node's rows.
+// Gather scheme, pass 2: one thread per matrix row.  A block owns a run of
+// consecutive nodes (thread t: node n0 + t/DIM, component t%DIM).  When the
+// free rows of those nodes form one contiguous CSR segment [s0, s1) the
+// segment is accumulated in LDS (each thread owns its row, so no two threads
+// touch one entry) and written back with coalesced stores; otherwise
+// (s0 == s1) the rows are accumulated in place in HBM.  Prescribed rows, only
+// assembled with all_rows, are always accumulated in place.  Each row sums its
+// incident elements in element order, the reference's summation order.
 template <int DIM, bool DO_K, bool DO_F, typename KT>
-__global__ __launch_bounds__(256) void k_gather(AsmArgs a) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= a.n_all) return;
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
+    extern __shared__ double s_seg[];
+    const int blk = blockIdx.x;
+    const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
+    const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
+    const bool use_lds = DO_K && s1 > s0;
+    const int t = threadIdx.x;
+    if (use_lds)
+        for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) s_seg[j] = 0.0;
+    const int64_t n = n0 + t / DIM;
+    const int da = t % DIM;
+    const bool active = n < n1;
     constexpr int S = DIM == 2 ? 3 : 6;
     constexpr int NQ = DIM == 2 ? 4 : 1;
-    int rr[DIM];
-    bool own[DIM], any = false;
-    for (int d = 0; d < DIM; ++d) {
-        rr[d] = a.dof[d * a.n_all + n];
-        own[d] = rr[d] < a.n_free || a.all_rows;
-        any |= own[d];
-    }
-    if (!any) return;
-    if (DO_K)
-        for (int d = 0; d < DIM; ++d)
-            if (own[d])
-                for (int32_t j = a.rowptr[rr[d]]; j < a.rowptr[rr[d] + 1]; ++j) {
-                    if (sizeof(KT) == 8) a.K[j] = 0.0;
-                    else a.K32[j] = 0.0f;
+    const int r = active ? a.dof[da * a.n_all + n] : 0;
+    const bool own = active && (r < a.n_free || a.all_rows);
+    const bool in_lds = use_lds && r < a.n_free;
+    if (DO_K && own && !in_lds)
+        for (int32_t j = a.rowptr[r]; j < a.rowptr[r + 1]; ++j) {
+            if (sizeof(KT) == 8) a.K[j] = 0.0;
+            else a.K32[j] = 0.0f;
+        }
+    __syncthreads();
+    if (own) {
+        const int64_t base = a.rowptr[r];
+        auto add = [&](int64_t pos, double v) {
+            if (in_lds) s_seg[pos - s0] += v;
+            else kadd<KT>(a, pos, v);
+        };
+        double Pa = 0.0;
+        for (int64_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
+            const int e = a.inc_elem[k];
+            const int la = a.inc_slot[k];
+            const int c = a.cut_index[e];
+            if (c < 0) {
+                int nd[4];
+                double X[4][DIM];
+                for (int q = 0; q < 4; ++q) {
+                    nd[q] = a.el_conn[(int64_t)q * a.n_elem + e];
+                    for (int i = 0; i < DIM; ++i) X[q][i] = a.X[i * a.n_all + nd[q]];
                 }
-    double Pa[DIM];
-    for (int d = 0; d < DIM; ++d) Pa[d] = 0.0;
-    for (int64_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
-        const int e = a.inc_elem[k];
-        const int la = a.inc_slot[k];
-        const int c = a.cut_index[e];
-        if (c < 0) {
-            int nd[4];
-            double X[4][DIM];
-            for (int q = 0; q < 4; ++q) {
-                nd[q] = a.el_conn[(int64_t)q * a.n_elem + e];
-                for (int i = 0; i < DIM; ++i) X[q][i] = a.X[i * a.n_all + nd[q]];
-            }
-            const int mat = a.el_mat[e];
-            double D[S * S];
-            for (int q = 0; q < S * S; ++q) D[q] = a.Dmat[mat * 36 + q];
-            double g[NQ][4 * DIM], w[NQ];
-            uncut_geometry<DIM>(X, a.tet_scale, g, w);
-            if (DO_F) {
-                double u[4 * DIM];
-                for (int q = 0; q < 4; ++q)
-                    for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
-                for (int q = 0; q < NQ; ++q) {
-                    double eps[S], sig[S];
-                    strain_of<DIM>(4, g[q], u, eps);
-                    stress_of<DIM>(D, eps, sig);
-                    for (int d = 0; d < DIM; ++d) Pa[d] += w[q] * force_entry<DIM>(&g[q][la * DIM], d, sig);
+                const int mat = a.el_mat[e];
+                double D[S * S];
+                for (int q = 0; q < S * S; ++q) D[q] = a.Dmat[mat * 36 + q];
+                double g[NQ][4 * DIM], w[NQ];
+                uncut_geometry<DIM>(X, a.tet_scale, g, w);
+                if (DO_F) {
+                    double u[4 * DIM];
+                    for (int q = 0; q < 4; ++q)
+                        for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
+                    for (int q = 0; q < NQ; ++q) {
+                        double eps[S], sig[S];
+                        strain_of<DIM>(4, g[q], u, eps);
+                        stress_of<DIM>(D, eps, sig);
+                        Pa += w[q] * force_entry<DIM>(&g[q][la * DIM], da, sig);
+                    }
                 }
-            }
-            if (DO_K) {
-                const uint64_t off = a.inc_off[k];
-                int cd[4][DIM];
-                for (int q = 0; q < 4; ++q)
-                    for (int d = 0; d < DIM; ++d) cd[q][d] = a.dof[d * a.n_all + nd[q]];
-                for (int da = 0; da < DIM; ++da) {
-                    if (!own[da]) continue;
+                if (DO_K) {
+                    const uint64_t off = a.inc_off[k];
                     double Kr[4 * DIM];
                     for (int j = 0; j < 4 * DIM; ++j) Kr[j] = 0.0;
                     for (int q = 0; q < NQ; ++q) {
                         double parts[S];
                         k_parts<DIM>(&g[q][la * DIM], da, D, parts);
                         for (int bn = 0; bn < 4; ++bn)
-                            for (int db = 0; db < DIM; ++db)
-                                Kr[bn * DIM + db] += w[q] * k_entry<DIM>(parts, &g[q][bn * DIM], db);
+                            for (int db = 0; db < DIM; ++db) Kr[bn * DIM + db] += w[q] * k_entry<DIM>(parts, &g[q][bn * DIM], db);
                     }
-                    const int64_t base = a.rowptr[rr[da]];
                     for (int bn = 0; bn < 4; ++bn) {
                         const int of = (int)((off >> (16 * bn)) & 0xff), op = (int)((off >> (16 * bn + 8)) & 0xff);
                         int cf = 0, cp = 0;
                         for (int db = 0; db < DIM; ++db) {
-                            const int64_t pos = cd[bn][db] < a.n_free ? base + of + cf++ : base + op + cp++;
-                            kadd<KT>(a, pos, Kr[bn * DIM + db]);
+                            const int cdof = a.dof[db * a.n_all + nd[bn]];
+                            const int64_t pos = cdof < a.n_free ? base + of + cf++ : base + op + cp++;
+                            add(pos, Kr[bn * DIM + db]);
                         }
                     }
                 }
-            }
-        } else {
-            const int nt = a.ce_nt[c], M = nt * DIM;
-            if (DO_F)
-                for (int d = 0; d < DIM; ++d) Pa[d] += a.pe[a.pe_ptr[c] + la * DIM + d];
-            if (DO_K) {
-                const double* ke = a.ke + a.ke_ptr[c];
-                const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
-                for (int da = 0; da < DIM; ++da) {
-                    if (!own[da]) continue;
-                    const int64_t base = a.rowptr[rr[da]];
-                    const double* krow = ke + (int64_t)(la * DIM + da) * M;
+            } else {
+                const int nt = a.ce_nt[c], M = nt * DIM;
+                if (DO_F) Pa += a.pe[a.pe_ptr[c] + la * DIM + da];
+                if (DO_K) {
+                    const double* krow = a.ke + a.ke_ptr[c] + (int64_t)(la * DIM + da) * M;
+                    const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
                     for (int bn = 0; bn < nt; ++bn) {
                         const int nb = a.ce_nodes[c * 16 + bn];
                         const int of = off[(la * nt + bn) * 2], op = off[(la * nt + bn) * 2 + 1];
@@ -483,16 +508,21 @@ __global__ __launch_bounds__(256) void k_gather(AsmArgs a) {
                         for (int db = 0; db < DIM; ++db) {
                             const int cdof = a.dof[db * a.n_all + nb];
                             const int64_t pos = cdof < a.n_free ? base + of + cf++ : base + op + cp++;
-                            kadd<KT>(a, pos, krow[bn * DIM + db]);
+                            add(pos, krow[bn * DIM + db]);
                         }
                     }
                 }
             }
         }
+        if (DO_F) a.P[r] = Pa;
     }
-    if (DO_F)
-        for (int d = 0; d < DIM; ++d)
-            if (own[d]) a.P[rr[d]] = Pa[d];
+    if (use_lds) {
+        __syncthreads();
+        for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) {
+            if (sizeof(KT) == 8) a.K[s0 + j] = s_seg[j];
+            else a.K32[s0 + j] = (float)s_seg[j];
+        }
+    }
 }
 
 }  // namespace pgfk
@@ -522,8 +552,9 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     if (c.n_cut > 0)
         hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)c.n_cut), dim3(pgfk::WAVE), 0, c.stream, a);
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
-    const unsigned nb = (unsigned)((c.n_all + 255) / 256);
-    hipLaunchKernelGGL((pgfk::k_gather<DIM, DO_K, DO_F, KT>), dim3(nb), dim3(256), 0, c.stream, a);
+    const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
+    hipLaunchKernelGGL((pgfk::k_gather<DIM, DO_K, DO_F, KT>), dim3((unsigned)c.n_gblk), dim3(pgfk::GATHER_THREADS), lds,
+                       c.stream, a);
     if (c.timing_events) (void)hipEventRecord(c.timing_events[2], c.stream);
 }
 

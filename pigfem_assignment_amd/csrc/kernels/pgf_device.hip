@@ -63,6 +63,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_off = inc_off;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
+    a.blk_node = blk_node; a.blk_seg = blk_seg;
     return a;
 }
 
@@ -72,7 +73,7 @@ Ctx::~Ctx() {
                     dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, r32, x32, z32, p032, p132, q32, dinv32, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
-                    inc_off, ke, pe};
+                    inc_off, ke, pe, blk_node, blk_seg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -508,6 +509,12 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             }
         for (int64_t q = 0; q < n_cpt; ++q)
             if (d->cpt_law[q] < 0 || d->cpt_law[q] >= d->n_coh_law) throw pgf::Error(PGF_EARG, "cohesive law out of range");
+        for (int64_t k = 0; k < d->n_cut; ++k) {
+            if (d->vpt_ptr[k + 1] - d->vpt_ptr[k] > pgfk::max_vpt(d->dim))
+                throw pgf::Error(PGF_ELIMIT, "cut element with more volumetric integration points than the kernel stages");
+            if (d->cpt_ptr[k + 1] - d->cpt_ptr[k] > pgfk::MAX_CPT)
+                throw pgf::Error(PGF_ELIMIT, "cut element with more cohesive integration points than the kernel stages");
+        }
         for (int64_t k = 0; k < d->n_cut; ++k)
             for (int64_t q = d->cpt_ptr[k]; q < d->cpt_ptr[k + 1]; ++q) {
                 const int nh = d->cpt_nh[q];
@@ -581,6 +588,40 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->pe_ptr = pgfd::dupload(pp.data(), d->n_cut + 1);
             c->ke = pgfd::dalloc<double>(kp[d->n_cut]);
             c->pe = pgfd::dalloc<double>(pp[d->n_cut]);
+            // gather blocks: runs of consecutive nodes, one thread per row;
+            // a block's free rows are staged in LDS when they form one
+            // contiguous CSR segment of at most SEG_MAX entries
+            const int per_blk = pgfk::GATHER_THREADS / d->dim;
+            std::vector<int64_t> bn{0}, bs;
+            int64_t seg_max = 0;
+            for (int64_t n0 = 0; n0 < d->n_all_nodes;) {
+                int64_t n1 = n0, rmin = INT64_MAX, rmax = -1, cnt = 0;
+                while (n1 < d->n_all_nodes && n1 - n0 < per_blk) {
+                    int64_t mn = rmin, mx = rmax, ct = cnt;
+                    for (int dd = 0; dd < d->dim; ++dd) {
+                        const int64_t g = d->dof[(size_t)dd * d->n_all_nodes + n1];
+                        if (g < d->n_free) { mn = std::min(mn, g); mx = std::max(mx, g); ++ct; }
+                    }
+                    if (n1 > n0 && ct > 0 && pat.rowptr[mx + 1] - pat.rowptr[mn] > pgfk::SEG_MAX) break;
+                    rmin = mn; rmax = mx; cnt = ct;
+                    ++n1;
+                }
+                int64_t s0 = 0, s1 = 0;
+                if (cnt > 0 && rmax - rmin + 1 == cnt && pat.rowptr[rmax + 1] - pat.rowptr[rmin] <= pgfk::SEG_MAX) {
+                    s0 = pat.rowptr[rmin];
+                    s1 = pat.rowptr[rmax + 1];
+                    seg_max = std::max(seg_max, s1 - s0);
+                }
+                bs.push_back(s0);
+                bs.push_back(s1);
+                bn.push_back(n1);
+                n0 = n1;
+            }
+            c->n_gblk = (int64_t)bn.size() - 1;
+            c->seg_max = seg_max;
+            if (c->n_gblk > 0x7fffffff) throw pgf::Error(PGF_ELIMIT, "too many gather blocks");
+            c->blk_node = pgfd::dupload(bn.data(), (int64_t)bn.size());
+            c->blk_seg = pgfd::dupload(bs.data(), std::max<int64_t>((int64_t)bs.size(), 2));
         }
         c->dmax_conv = pgfd::dalloc<double>(n_cpt);
         c->dmax_trial = pgfd::dalloc<double>(n_cpt);

@@ -44,6 +44,8 @@ struct Ctx {
     uint64_t* inc_off = nullptr;
     double *ke = nullptr, *pe = nullptr;
     int64_t n_inc = 0, n_ke = 0;
+    int64_t *blk_node = nullptr, *blk_seg = nullptr;
+    int64_t n_gblk = 0, seg_max = 0;
     // free / prescribed dof -> nodal field slot (d*n_all + node)
     int64_t *free_slot = nullptr, *pres_slot = nullptr;
     // matrix

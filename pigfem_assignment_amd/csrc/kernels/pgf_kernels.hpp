@@ -20,6 +20,12 @@ namespace pgfk {
 constexpr int WAVE = 64;
 constexpr int MAX_NT = 12;          // element slots of a cut element
 constexpr int MAX_M = MAX_NT * 3;   // element dofs
+// integration points of one cut element staged in LDS (checked at create)
+constexpr int max_vpt(int dim) { return dim == 2 ? 32 : 16; }
+constexpr int MAX_CPT = 8;
+// row-gather blocks: threads (one per matrix row) and LDS segment (doubles)
+constexpr int GATHER_THREADS = 128;
+constexpr int SEG_MAX = 4096;
 
 struct CohParams {
     int type;
@@ -74,6 +80,8 @@ struct AsmArgs {
     const int64_t* ke_ptr;
     double* pe;                // cut-element P_e scratch
     const int64_t* pe_ptr;
+    const int64_t* blk_node;   // [n_gblk+1] first node of each gather block
+    const int64_t* blk_seg;    // [n_gblk][2] CSR segment of the block's free rows (empty: not staged)
     // outputs
     const int32_t* rowptr;
     double* K;
