@@ -41,6 +41,31 @@ __device__ inline void k_parts(const double* g, int da, const double* D, double*
             for (int k = 0; k < 6; ++k) parts[k] = g[2] * D[2 * 6 + k] + g[1] * D[3 * 6 + k] + g[0] * D[4 * 6 + k];
     }
 }
+// The same parts with the node's Voigt B column built first (zeros included)
+// and multiplied through in row order; the nonzero terms are summed in the
+// order k_parts uses.  Used where da is a run-time value inside a loop.
+template <int DIM>
+__device__ inline void k_parts_col(const double* g, int da, const double* D, double* parts) {
+    constexpr int S = DIM == 2 ? 3 : 6;
+    double b[S];
+    if (DIM == 2) {
+        b[0] = da == 0 ? g[0] : 0.0;
+        b[1] = da == 1 ? g[1] : 0.0;
+        b[2] = da == 0 ? g[1] : g[0];
+    } else {
+        b[0] = da == 0 ? g[0] : 0.0;
+        b[1] = da == 1 ? g[1] : 0.0;
+        b[2] = da == 2 ? g[2] : 0.0;
+        b[3] = da == 0 ? 0.0 : (da == 1 ? g[2] : g[1]);
+        b[4] = da == 1 ? 0.0 : (da == 0 ? g[2] : g[0]);
+        b[5] = da == 2 ? 0.0 : (da == 0 ? g[1] : g[0]);
+    }
+    for (int k = 0; k < S; ++k) {
+        double s = 0.0;
+        for (int l = 0; l < S; ++l) s += b[l] * D[l * S + k];
+        parts[k] = s;
+    }
+}
 // K(row, (b,db)) from parts and column node gradient gb
 template <int DIM>
 __device__ inline double k_entry(const double* parts, const double* gb, int db) {
@@ -228,7 +253,7 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
 // ---------------------------------------------------------------------------
 // Cut element computed by one wavefront into LDS: K_e (M x M) and P_e (M).
 // Every integration point is staged in LDS first (gradients, J*W, the point's
-// D, its stress; per cohesive point the rotated tangent R^T D_c R, the rotated
+// material, its stress; per cohesive point the rotated tangent R^T D_c R, the rotated
 // traction and the surface shape functions), then each lane owns a set of
 // K_e entries and sums them over the points in point order (volumetric points,
 // then cohesive points): three barriers per element.  The cohesive law runs
@@ -244,7 +269,7 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
     __shared__ double s_u[MM];
     __shared__ double s_g[NV * MM];
     __shared__ double s_w[NV];
-    __shared__ double s_D[NV * S * S];
+    __shared__ int s_vmat[NV];
     __shared__ double s_sig[NV * S];
     __shared__ double s_cRDR[MAX_CPT * DIM * DIM], s_ctR[MAX_CPT * DIM], s_cN[MAX_CPT * 4], s_cw[MAX_CPT];
     __shared__ int s_cmap[MAX_CPT * 8], s_cnh[MAX_CPT];
@@ -254,24 +279,23 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
     const int nc = (int)(a.cpt_ptr[c + 1] - c0);
     if (tid < nt) s_node[tid] = a.ce_nodes[c * 16 + tid];
     __syncthreads();
-    for (int i = tid; i < M; i += WAVE) {
+    for (int i = tid; i < M; i += CUT_THREADS) {
         s_dof[i] = a.dof[(i % DIM) * a.n_all + s_node[i / DIM]];
         s_u[i] = a.U[(i % DIM) * a.n_all + s_node[i / DIM]];
     }
-    for (int k = tid; k < nv * M; k += WAVE) {
+    for (int k = tid; k < nv * M; k += CUT_THREADS) {
         const int q = k / M;
         s_g[q * MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
     }
-    for (int k = tid; k < nv * S * S; k += WAVE) {
-        const int q = k / (S * S), l = k - q * S * S;
-        s_D[k] = a.Dmat[a.vpt_mat[v0 + q] * 36 + l];
+    if (tid < nv) {
+        s_w[tid] = a.vpt_w[v0 + tid];
+        s_vmat[tid] = a.vpt_mat[v0 + tid];
     }
-    if (tid < nv) s_w[tid] = a.vpt_w[v0 + tid];
     __syncthreads();
     if (DO_F && tid < nv) {
         double eps[S];
         strain_of<DIM>(nt, &s_g[tid * MM], s_u, eps);
-        stress_of<DIM>(&s_D[tid * S * S], eps, &s_sig[tid * S]);
+        stress_of<DIM>(a.Dmat + s_vmat[tid] * 36, eps, &s_sig[tid * S]);
     }
     if (tid < nc) {
         const int64_t q = c0 + tid;
@@ -315,19 +339,23 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
     }
     __syncthreads();
     if (DO_K)
-        for (int e = tid; e < M * M; e += WAVE) {
+        for (int e = tid; e < M * M; e += CUT_THREADS) {
             const int i = e / M, j = e - (e / M) * M;
             const int an = i / DIM, da = i % DIM, bn = j / DIM, db = j % DIM;
             double acc = 0.0;
+            #pragma unroll 1
             for (int q = 0; q < nv; ++q) {
                 double parts[S];
-                k_parts<DIM>(&s_g[q * MM + an * DIM], da, &s_D[q * S * S], parts);
+                k_parts_col<DIM>(&s_g[q * MM + an * DIM], da, a.Dmat + s_vmat[q] * 36, parts);
                 acc += s_w[q] * k_entry<DIM>(parts, &s_g[q * MM + bn * DIM], db);
             }
+            #pragma unroll 1
             for (int q = 0; q < nc; ++q) {
                 const int nh = s_cnh[q];
+                #pragma unroll 1
                 for (int k1 = 0; k1 < 2 * nh; ++k1) {
                     if (s_cmap[q * 8 + k1] != an) continue;
+                    #pragma unroll 1
                     for (int k2 = 0; k2 < 2 * nh; ++k2) {
                         if (s_cmap[q * 8 + k2] != bn) continue;
                         const double v = s_cN[q * 4 + k1 % nh] * s_cN[q * 4 + k2 % nh] * s_cRDR[q * DIM * DIM + da * DIM + db];
@@ -339,12 +367,15 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
             s_K[e] = acc;
         }
     if (DO_F)
-        for (int i = tid; i < M; i += WAVE) {
+        for (int i = tid; i < M; i += CUT_THREADS) {
             const int an = i / DIM, d = i % DIM;
             double acc = 0.0;
+            #pragma unroll 1
             for (int q = 0; q < nv; ++q) acc += s_w[q] * force_entry<DIM>(&s_g[q * MM + an * DIM], d, &s_sig[q * S]);
+            #pragma unroll 1
             for (int q = 0; q < nc; ++q) {
                 const int nh = s_cnh[q];
+                #pragma unroll 1
                 for (int k = 0; k < 2 * nh; ++k)
                     if (s_cmap[q * 8 + k] == an) {
                         const double v = s_cN[q * 4 + k % nh] * s_ctR[q * DIM + d];
@@ -358,7 +389,7 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
 
 // Coloured scheme, cut elements: compute then scatter into the CSR rows.
 template <int DIM, bool DO_K, bool DO_F, typename KT>
-__global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
+__global__ __launch_bounds__(CUT_THREADS) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
     const int64_t li = begin + blockIdx.x;
     if (li >= end) return;
     constexpr int MM = MAX_NT * DIM;
@@ -370,12 +401,12 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
     cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
     const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
     if (DO_F)
-        for (int i = tid; i < M; i += WAVE) {
+        for (int i = tid; i < M; i += CUT_THREADS) {
             const int r = s_dof[i];
             if (r < a.n_free || a.all_rows) a.P[r] += s_P[i];
         }
     if (DO_K)
-        for (int e = tid; e < M * M; e += WAVE) {
+        for (int e = tid; e < M * M; e += CUT_THREADS) {
             const int i = e / M, j = e % M;
             const int r = s_dof[i];
             if (r >= a.n_free && !a.all_rows) continue;
@@ -391,7 +422,7 @@ __global__ __launch_bounds__(WAVE) void k_cut(AsmArgs a, int64_t begin, int64_t 
 
 // Gather scheme, pass 1: every cut element's K_e / P_e into the dense scratch.
 template <int DIM, bool DO_K, bool DO_F>
-__global__ __launch_bounds__(WAVE) void k_cut_elem(AsmArgs a) {
+__global__ __launch_bounds__(CUT_THREADS) void k_cut_elem(AsmArgs a) {
     constexpr int MM = MAX_NT * DIM;
     const int c = blockIdx.x;
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
@@ -401,11 +432,11 @@ __global__ __launch_bounds__(WAVE) void k_cut_elem(AsmArgs a) {
     cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
     if (DO_K) {
         double* ke = a.ke + a.ke_ptr[c];
-        for (int e = tid; e < M * M; e += WAVE) ke[e] = s_K[e];
+        for (int e = tid; e < M * M; e += CUT_THREADS) ke[e] = s_K[e];
     }
     if (DO_F) {
         double* pe = a.pe + a.pe_ptr[c];
-        for (int i = tid; i < M; i += WAVE) pe[i] = s_P[i];
+        for (int i = tid; i < M; i += CUT_THREADS) pe[i] = s_P[i];
     }
 }
 
@@ -419,7 +450,7 @@ __global__ __launch_bounds__(WAVE) void k_cut_elem(AsmArgs a) {
 // incident elements in element order, the reference's summation order.
 template <int DIM, bool DO_K, bool DO_F, typename KT>
 __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
-    extern __shared__ double s_seg[];
+    HIP_DYNAMIC_SHARED(double, s_seg)
     const int blk = blockIdx.x;
     const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
     const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
@@ -541,7 +572,7 @@ static void launch_colors(const Ctx& c, const pgfk::AsmArgs& a) {
         }
         int64_t cb = c.ce_color_ptr[col], ce = c.ce_color_ptr[col + 1];
         if (ce > cb)
-            hipLaunchKernelGGL((pgfk::k_cut<DIM, DO_K, DO_F, KT>), dim3((unsigned)(ce - cb)), dim3(pgfk::WAVE), 0, c.stream, a,
+            hipLaunchKernelGGL((pgfk::k_cut<DIM, DO_K, DO_F, KT>), dim3((unsigned)(ce - cb)), dim3(pgfk::CUT_THREADS), 0, c.stream, a,
                                cb, ce);
     }
 }
@@ -550,7 +581,7 @@ template <int DIM, bool DO_K, bool DO_F, typename KT>
 static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     if (c.timing_events) (void)hipEventRecord(c.timing_events[0], c.stream);
     if (c.n_cut > 0)
-        hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)c.n_cut), dim3(pgfk::WAVE), 0, c.stream, a);
+        hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)c.n_cut), dim3(pgfk::CUT_THREADS), 0, c.stream, a);
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
     hipLaunchKernelGGL((pgfk::k_gather<DIM, DO_K, DO_F, KT>), dim3((unsigned)c.n_gblk), dim3(pgfk::GATHER_THREADS), lds,

@@ -23,6 +23,8 @@ constexpr int MAX_M = MAX_NT * 3;   // element dofs
 // integration points of one cut element staged in LDS (checked at create)
 constexpr int max_vpt(int dim) { return dim == 2 ? 32 : 16; }
 constexpr int MAX_CPT = 8;
+// threads of the cut-element kernels (one workgroup per cut element)
+constexpr int CUT_THREADS = 64;
 // row-gather blocks: threads (one per matrix row) and LDS segment (doubles)
 constexpr int GATHER_THREADS = 128;
 constexpr int SEG_MAX = 4096;
