@@ -101,6 +101,10 @@ EXPORTS = [
     "pgfh_problem_create", "pgfh_problem_destroy", "pgfh_last_error", "pgfh_problem_info", "pgfh_mesh_desc",
     "pgfh_get_coords", "pgfh_get_conn", "pgfh_get_elem_mat", "pgfh_get_node_detect", "pgfh_get_dofs",
     "pgfh_get_enrich_edges", "pgfh_get_cut", "pgfh_pattern_info", "pgfh_get_pattern", "pgfh_solve_load_steps",
+    "pgf_net_create", "pgf_net_destroy", "pgf_net_last_error", "pgf_net_num_params", "pgf_net_get_params",
+    "pgf_net_set_params", "pgf_net_forward", "pgf_net_backward", "pgf_net_add_sample", "pgf_net_num_samples",
+    "pgf_net_train", "pgf_learn_attach", "pgf_learn_detach", "pgf_learn_commit", "pgf_learn_record", "pgf_learn_last",
+    "pgf_learn_features",
 ]
 
 _lib = None
@@ -128,6 +132,13 @@ def lib():
     L.pgf_time_assemble_split.argtypes = [C.c_void_p, C.c_uint32, C.c_int32, dp]
     L.pgf_time_spmv.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
     L.pgfh_problem_create.argtypes = [C.POINTER(HostConfig), C.POINTER(C.c_void_p)]
+    L.pgf_net_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+    L.pgf_net_destroy.argtypes = [C.c_void_p]
+    L.pgf_net_destroy.restype = None
+    L.pgf_net_last_error.argtypes = [C.c_void_p]
+    L.pgf_net_last_error.restype = C.c_char_p
+    L.pgf_net_train.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.POINTER(C.c_float)]
+    L.pgf_learn_attach.argtypes = [C.c_void_p, C.c_void_p, dp, C.POINTER(C.c_float)]
     L.pgfh_solve_load_steps.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(LoadOpts), STEP_CB, C.c_void_p,
                                         C.POINTER(LoadInfo)]
     for name in ["pgf_destroy", "pgf_csr_info_get", "pgf_get_csr", "pgf_get_colors", "pgf_set_solution",
@@ -149,8 +160,11 @@ def ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
-def check(rc, ctx=None, host=False):
+def check(rc, ctx=None, host=False, net=None):
     if rc != 0:
         L = lib()
-        msg = (L.pgfh_last_error() if host else L.pgf_last_error(ctx)) or b""
+        if net is not None:
+            msg = L.pgf_net_last_error(net) or b""
+        else:
+            msg = (L.pgfh_last_error() if host else L.pgf_last_error(ctx)) or b""
         raise RuntimeError("pgfem error %d: %s" % (rc, msg.decode(errors="replace")))

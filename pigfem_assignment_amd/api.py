@@ -264,6 +264,30 @@ class DeviceContext:
         self._chk(self.lib.pgf_time_spmv(self.h, precision, reps, C.byref(ms)))
         return ms.value
 
+    # ---- learned initial guess (include/pgfem_learn.h) ----
+    def learn_attach(self, net=None, box=(0, 0, 0, 1, 1, 1), indicator=None):
+        b = np.ascontiguousarray(box, np.float64)
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float32).ravel()
+        self._learn_net = net  # keep the network alive while attached
+        self._chk(self.lib.pgf_learn_attach(self.h, net.h if net is not None else None, L.ptr(b, C.c_double),
+                                            L.ptr(ind, C.c_float)))
+
+    def learn_commit(self):
+        self._chk(self.lib.pgf_learn_commit(self.h))
+
+    def learn_record(self):
+        self._chk(self.lib.pgf_learn_record(self.h))
+
+    def learn_last(self):
+        out = np.zeros(6)
+        self._chk(self.lib.pgf_learn_last(self.h, L.ptr(out, C.c_double)))
+        return dict(alpha=out[:4].copy(), cols=int(out[4]), ms=out[5])
+
+    def learn_features(self, grid=64):
+        f = np.zeros((grid, grid, grid, 8), np.float32)
+        self._chk(self.lib.pgf_learn_features(self.h, L.ptr(f, C.c_float)))
+        return f
+
     def solve_load_steps(self, T_final=1.0, max_time_step=0.05, min_time_step=1e-6, max_repeats=20, max_steps=0,
                          callback=None, **newton):
         nw = dict(rel_tol=1e-8, abs_tol=0.0, max_iter=40, precision=64, lin_rtol=1e-10, lin_maxit=50000,
@@ -287,3 +311,73 @@ class DeviceContext:
         if rc != 0:
             raise RuntimeError("load stepping failed (%d): %s" % (rc, self.lib.pgfh_last_error().decode()))
         return info, records
+
+
+class GuessNet:
+    """The learned-guess network (include/pgfem_learn.h) on the device."""
+
+    GRID = 64
+    IN_CH = 8
+    OUT_CH = 3
+
+    def __init__(self, seed=1):
+        self.lib = L.lib()
+        h = C.c_void_p()
+        L.check(self.lib.pgf_net_create(C.c_uint64(seed), C.byref(h)), net=None)
+        self.h = h
+        n = C.c_int64()
+        self._chk(self.lib.pgf_net_num_params(self.h, C.byref(n)))
+        self.n_params = n.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pgf_net_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        L.check(rc, net=self.h)
+
+    def params(self):
+        w = np.zeros(self.n_params, np.float32)
+        self._chk(self.lib.pgf_net_get_params(self.h, L.ptr(w, C.c_float)))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.n_params
+        self._chk(self.lib.pgf_net_set_params(self.h, L.ptr(w, C.c_float)))
+
+    def _x(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        assert x.shape == (self.GRID,) * 3 + (self.IN_CH,)
+        return x
+
+    def forward(self, x):
+        x = self._x(x)
+        y = np.zeros((self.GRID,) * 3 + (self.OUT_CH,), np.float32)
+        self._chk(self.lib.pgf_net_forward(self.h, L.ptr(x, C.c_float), L.ptr(y, C.c_float)))
+        return y
+
+    def backward(self, x, dy):
+        x = self._x(x)
+        dy = np.ascontiguousarray(dy, np.float32)
+        assert dy.shape == (self.GRID,) * 3 + (self.OUT_CH,)
+        g = np.zeros(self.n_params, np.float32)
+        self._chk(self.lib.pgf_net_backward(self.h, L.ptr(x, C.c_float), L.ptr(dy, C.c_float), L.ptr(g, C.c_float)))
+        return g
+
+    def add_sample(self, x, target):
+        x = self._x(x)
+        t = np.ascontiguousarray(target, np.float32)
+        assert t.shape == (self.GRID,) * 3 + (self.OUT_CH,)
+        self._chk(self.lib.pgf_net_add_sample(self.h, L.ptr(x, C.c_float), L.ptr(t, C.c_float)))
+
+    def num_samples(self):
+        n = C.c_int32()
+        self._chk(self.lib.pgf_net_num_samples(self.h, C.byref(n)))
+        return n.value
+
+    def train(self, epochs, lr=1e-3):
+        loss = (C.c_float * max(epochs, 1))()
+        self._chk(self.lib.pgf_net_train(self.h, int(epochs), float(lr), loss))
+        return np.array(loss[:epochs], np.float64)

@@ -77,6 +77,7 @@ Ctx::~Ctx() {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
+    if (learn) learn_free(learn);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -394,6 +395,12 @@ int guard(Ctx* c, F&& f) {
 extern "C" {
 
 struct pgf_ctx : public Ctx {};
+
+}  // extern "C"
+
+pgfd::Ctx* pgfd::ctx_of(pgf_ctx* h) { return h; }
+
+extern "C" {
 
 PGF_API const char* pgf_last_error(const pgf_ctx* ctx) {
     return ctx ? ctx->err.c_str() : pgfd::g_last_error.c_str();
@@ -909,7 +916,8 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
             HIPCHK(hipEventElapsedTime(&ms, a0, a1));
             ms_asm += ms;
             pgf_solve_info si{};
-            const bool g = o->use_guess && c->have_guess && it == 0;
+            bool g = o->use_guess == 1 && c->have_guess && it == 0;
+            if (it == 0 && c->learn) g = pgfd::learn_step_start(*c, o->use_guess, fp32) || g;
             if (fp32) {
                 if (g) hipLaunchKernelGGL(pgfd::k_to_f32, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x32);
                 pgfd::run_pcg<float>(*c, c->K32, c->r, true, c->x32, c->r32, c->z32, c->p032, c->p132, c->q32, c->dinv32, g,

@@ -8,7 +8,11 @@
 
 #include "pgf_kernels.hpp"
 
+struct pgf_ctx;  // the C-ABI handle (derives from pgfd::Ctx)
+
 namespace pgfd {
+
+struct LearnState;  // learned initial guess state (pgf_cnn.hip)
 
 struct Ctx {
     std::string err;
@@ -67,6 +71,7 @@ struct Ctx {
     double* h_scal = nullptr;      // pinned mirror
     bool have_guess = false;
     hipEvent_t* timing_events = nullptr;  // when set: cut pass start, gather start, gather end
+    LearnState* learn = nullptr;
 
     pgfk::AsmArgs asm_args() const;
     ~Ctx();
@@ -75,5 +80,14 @@ struct Ctx {
 template <typename T>
 void fill_async(Ctx& c, T* p, int64_t n, T v);
 void enqueue_assembly(Ctx& c, double dt, uint32_t flags);
+
+// Learned initial guess (pgf_cnn.hip).  Called once per Newton step after the
+// first Jacobian assembly, with the step residual in c.r: records the
+// step-start features and U; for use_guess 2 (network + previous increment)
+// or 3 (previous increment only) writes a Galerkin-scaled guess to c.guess
+// and returns true.
+bool learn_step_start(Ctx& c, int use_guess, bool fp32);
+void learn_free(LearnState* s);
+Ctx* ctx_of(pgf_ctx* h);
 
 }  // namespace pgfd

@@ -20,7 +20,7 @@ TET_EDGES = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
 
 def test_library_exports_every_declared_symbol(lib):
     declared = set()
-    for h in ("pgfem.h", "pgfem_host.h"):
+    for h in ("pgfem.h", "pgfem_host.h", "pgfem_learn.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         declared |= set(re.findall(r"PGF_API\s+[\w\s\*]+?\b(pgfh?_\w+)\s*\(", txt))
     assert declared == set(L.EXPORTS), declared ^ set(L.EXPORTS)

@@ -1,0 +1,79 @@
+"""Learned-guess network on the device vs the CPU oracle (oracle/cnn_oracle.py).
+
+Forward outputs and parameter gradients of the bf16-MFMA kernels are compared
+with the oracle's float64 evaluation at the same bf16 rounding points.  The
+remaining differences come from FP32 accumulation order and occasional
+one-ulp bf16 flips of activations, so the bar is normwise: ||gpu - oracle|| /
+||oracle|| <= 1e-2 for the output and for each layer's weight and bias
+gradients (the oracle itself is pinned by finite differences in
+tests/test_cpu_cnn_oracle.py).
+"""
+import numpy as np
+import pytest
+
+from oracle import cnn_oracle as N
+from pigfem_assignment_amd.api import GuessNet
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-2
+
+
+def _inputs(G, seed):
+    rng = np.random.RandomState(seed)
+    x = rng.standard_normal((G, G, G, 8)).astype(np.float32)
+    x[..., 0] = (x[..., 0] > 0.5)  # indicator-like channel
+    return x
+
+
+def _layer_slices():
+    out, o = [], 0
+    for m, ci, co in N.SPECS:
+        nw = N.taps(m) * ci * co
+        out.append((slice(o, o + nw), slice(o + nw, o + nw + co)))
+        o += nw + co
+    return out
+
+
+def test_forward_matches_oracle():
+    net = GuessNet(seed=11)
+    G = net.GRID
+    x = _inputs(G, 1)
+    y = net.forward(x)
+    yo = N.forward(net.params().astype(np.float64), x)[0][..., :3]
+    err = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    print("forward normwise rel err %.3e" % err)
+    assert err <= TOL
+
+
+def test_gradients_match_oracle():
+    net = GuessNet(seed=12)
+    G = net.GRID
+    x = _inputs(G, 2)
+    dy = np.random.RandomState(3).standard_normal((G, G, G, 3)).astype(np.float32)
+    g = net.backward(x, dy)
+    dy16 = np.zeros((G, G, G, 16))
+    dy16[..., :3] = dy
+    go = N.backward(net.params().astype(np.float64), x, dy16)
+    for l, (sw, sb) in enumerate(_layer_slices()):
+        ew = np.linalg.norm(g[sw] - go[sw]) / np.linalg.norm(go[sw])
+        eb = np.linalg.norm(g[sb] - go[sb]) / max(np.linalg.norm(go[sb]), 1e-30)
+        print("layer %d: weight grad rel err %.3e, bias grad rel err %.3e" % (l, ew, eb))
+        assert ew <= TOL and eb <= TOL
+
+
+def test_training_reduces_loss():
+    net = GuessNet(seed=13)
+    G = net.GRID
+    rng = np.random.RandomState(4)
+    for s in range(4):
+        x = _inputs(G, 10 + s)
+        # a smooth, learnable target: a local average of input channels 2..4
+        t = x[..., 2:5].copy()
+        for ax in range(3):
+            t = 0.5 * t + 0.25 * (np.roll(t, 1, axis=ax) + np.roll(t, -1, axis=ax))
+        net.add_sample(x, t.astype(np.float32))
+    loss = net.train(40, lr=2e-3)
+    print("loss by epoch:", np.array2string(loss[[0, 9, 19, 29, 39]], precision=4))
+    assert np.all(np.isfinite(loss))
+    assert loss[-1] < 0.5 * loss[0]
