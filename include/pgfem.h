@@ -190,6 +190,27 @@ PGF_API int pgf_pcg(pgf_ctx* ctx, const double* b, double* x, const double* x0, 
 /* optional initial guess for the next Newton solves, [n_free] (learned predictor) */
 PGF_API int pgf_set_guess(pgf_ctx* ctx, const double* dx);
 
+/* ---- batched solves of independent systems (RVEs x load cases) ----
+ * Replaces the per-RVE KSPSolve of the subscale loop (src/SubscaleModel.cpp:42-196)
+ * and the multi-right-hand-side solve of SensitivityDirect::solveProblem
+ * (MatMatSolve, src/SensitivityDirect.cpp:96-136).  K_ff of the context must be
+ * block diagonal over the free-row ranges sys_ptr[0..n_sys] (a context built on
+ * a concatenation of independent meshes; checked).  Every (system, rhs) pair
+ * runs its own Jacobi-PCG from a zero start to ||r|| <= rtol ||b||, all pairs
+ * in one captured-graph iteration sweep.  b, x: host [nrhs][n_free], nrhs 1..4;
+ * iters, relres: [n_sys][nrhs]; tune[0..1] in: threads per row (4/8/16) and rows
+ * per tile, 0 = autotune; out: the choice. */
+PGF_API int pgf_batch_pcg(pgf_ctx* ctx, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs, const double* b,
+                          double* x, double rtol, int32_t maxit, int32_t* iters, double* relres, double* ms,
+                          int32_t* tune);
+
+/* right-hand sides of prescribed-displacement load cases: b = -K_fp u_p
+ * (src/SolverLinear.cpp:128-129 per load case); up host [nrhs][n_pres], b host [nrhs][n_free] */
+PGF_API int pgf_rhs_prescribed(pgf_ctx* ctx, int32_t nrhs, const double* up, double* b);
+/* reactions R = K_pf u_f + K_pp u_p (src/SolverLinear.cpp:135-137); needs an
+ * assembly with PGF_ASM_ALL_ROWS; uf [nrhs][n_free], up [nrhs][n_pres], R [nrhs][n_pres] */
+PGF_API int pgf_reactions(pgf_ctx* ctx, int32_t nrhs, const double* uf, const double* up, double* R);
+
 /* ---- Newton load step on the device (residual FP64) ---- */
 PGF_API int pgf_newton_step(pgf_ctx* ctx, double frac, const pgf_newton_opts* opts, pgf_step_info* info);
 

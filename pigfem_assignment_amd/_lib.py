@@ -104,7 +104,7 @@ EXPORTS = [
     "pgf_net_create", "pgf_net_destroy", "pgf_net_last_error", "pgf_net_num_params", "pgf_net_get_params",
     "pgf_net_set_params", "pgf_net_forward", "pgf_net_backward", "pgf_net_add_sample", "pgf_net_num_samples",
     "pgf_net_train", "pgf_learn_attach", "pgf_learn_detach", "pgf_learn_commit", "pgf_learn_record", "pgf_learn_last",
-    "pgf_learn_features",
+    "pgf_learn_features", "pgf_batch_pcg", "pgf_rhs_prescribed", "pgf_reactions",
 ]
 
 _lib = None
@@ -138,6 +138,8 @@ def lib():
     L.pgf_net_last_error.argtypes = [C.c_void_p]
     L.pgf_net_last_error.restype = C.c_char_p
     L.pgf_net_train.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.POINTER(C.c_float)]
+    L.pgf_batch_pcg.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.c_int32, dp, dp, C.c_double, C.c_int32,
+                                C.POINTER(C.c_int32), dp, dp, C.POINTER(C.c_int32)]
     L.pgf_learn_attach.argtypes = [C.c_void_p, C.c_void_p, dp, C.POINTER(C.c_float)]
     L.pgfh_solve_load_steps.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(LoadOpts), STEP_CB, C.c_void_p,
                                         C.POINTER(LoadInfo)]

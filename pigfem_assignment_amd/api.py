@@ -264,6 +264,40 @@ class DeviceContext:
         self._chk(self.lib.pgf_time_spmv(self.h, precision, reps, C.byref(ms)))
         return ms.value
 
+    # ---- batched solves (pgf_batch_pcg) ----
+    def batch_pcg(self, sys_ptr, B, rtol=1e-10, maxit=100000, tune=(0, 0)):
+        """Solve K_ff X = B per system (free-row ranges sys_ptr) for the rows of
+        B [nrhs][n_free]; returns X, iterations [n_sys][nrhs], relres, ms, tune."""
+        sp = np.ascontiguousarray(sys_ptr, np.int64)
+        B = np.ascontiguousarray(B, np.float64)
+        nrhs = B.shape[0]
+        X = np.zeros_like(B)
+        n_sys = len(sp) - 1
+        its = np.zeros((n_sys, nrhs), np.int32)
+        rel = np.zeros((n_sys, nrhs))
+        ms = C.c_double()
+        tn = (C.c_int32 * 2)(int(tune[0]), int(tune[1]))
+        rc = self.lib.pgf_batch_pcg(self.h, n_sys, sp.ctypes.data_as(C.POINTER(C.c_int64)), nrhs, L.ptr(B, C.c_double),
+                                    L.ptr(X, C.c_double), rtol, maxit, its.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    L.ptr(rel, C.c_double), C.byref(ms), tn)
+        if rc not in (0, -5):
+            self._chk(rc)
+        return X, its, rel, ms.value, (tn[0], tn[1])
+
+    def rhs_prescribed(self, UP):
+        UP = np.ascontiguousarray(UP, np.float64)
+        B = np.zeros((UP.shape[0], self.csr.n_free))
+        self._chk(self.lib.pgf_rhs_prescribed(self.h, UP.shape[0], L.ptr(UP, C.c_double), L.ptr(B, C.c_double)))
+        return B
+
+    def reactions(self, UF, UP):
+        UF = np.ascontiguousarray(UF, np.float64)
+        UP = np.ascontiguousarray(UP, np.float64)
+        R = np.zeros_like(UP)
+        self._chk(self.lib.pgf_reactions(self.h, UF.shape[0], L.ptr(UF, C.c_double), L.ptr(UP, C.c_double),
+                                         L.ptr(R, C.c_double)))
+        return R
+
     # ---- learned initial guess (include/pgfem_learn.h) ----
     def learn_attach(self, net=None, box=(0, 0, 0, 1, 1, 1), indicator=None):
         b = np.ascontiguousarray(box, np.float64)

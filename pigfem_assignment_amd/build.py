@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpgfem.so")
 
 HOST_SRC = ["host/pgf_mesh.cpp", "host/pgf_dofs.cpp", "host/pgf_problem.cpp"]
-HIP_SRC = ["kernels/pgf_assemble.hip", "kernels/pgf_device.hip", "kernels/pgf_cnn.hip"]
+HIP_SRC = ["kernels/pgf_assemble.hip", "kernels/pgf_device.hip", "kernels/pgf_cnn.hip", "kernels/pgf_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

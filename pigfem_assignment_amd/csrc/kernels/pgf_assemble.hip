@@ -253,10 +253,11 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
 // ---------------------------------------------------------------------------
 // Cut element computed by one wavefront into LDS: K_e (M x M) and P_e (M).
 // Every integration point is staged in LDS first (gradients, J*W, the point's
-// material, its stress; per cohesive point the rotated tangent R^T D_c R, the rotated
-// traction and the surface shape functions), then each lane owns a set of
-// K_e entries and sums them over the points in point order (volumetric points,
-// then cohesive points): three barriers per element.  The cohesive law runs
+// material, its stress; per cohesive point the rotated tangent R^T D_c R, the
+// rotated traction and the surface shape functions), then each lane owns one
+// row of K_e (or a column group of it), forms that row's D B column once per
+// point and sums every point's term into its entries in point order
+// (volumetric points, then cohesive points): three barriers per element.  The cohesive law runs
 // once per cohesive point (lane q), which also writes that point's trial
 // delta_max and damage flags.
 template <int DIM, bool DO_K, bool DO_F>
@@ -338,34 +339,40 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
         s_cw[tid] = a.cpt_w[q];
     }
     __syncthreads();
-    if (DO_K)
-        for (int e = tid; e < M * M; e += CUT_THREADS) {
-            const int i = e / M, j = e - (e / M) * M;
-            const int an = i / DIM, da = i % DIM, bn = j / DIM, db = j % DIM;
-            double acc = 0.0;
-            #pragma unroll 1
+    if (DO_K) {
+        // lane -> (row i, column group grp): the lane owns K_e(i, j) for
+        // j = grp mod nsplit and adds every point's term to them in order
+        const int nsplit = CUT_THREADS / M;
+        if (tid < M * nsplit) {
+            const int i = tid % M, grp = tid / M;
+            const int an = i / DIM, da = i % DIM;
+            double* Ki = s_K + i * M;
+            for (int j = grp; j < M; j += nsplit) Ki[j] = 0.0;
             for (int q = 0; q < nv; ++q) {
                 double parts[S];
                 k_parts_col<DIM>(&s_g[q * MM + an * DIM], da, a.Dmat + s_vmat[q] * 36, parts);
-                acc += s_w[q] * k_entry<DIM>(parts, &s_g[q * MM + bn * DIM], db);
+                const double wq = s_w[q];
+                const double* gq = &s_g[q * MM];
+                for (int j = grp; j < M; j += nsplit) Ki[j] += wq * k_entry<DIM>(parts, gq + (j / DIM) * DIM, j % DIM);
             }
-            #pragma unroll 1
             for (int q = 0; q < nc; ++q) {
                 const int nh = s_cnh[q];
-                #pragma unroll 1
                 for (int k1 = 0; k1 < 2 * nh; ++k1) {
                     if (s_cmap[q * 8 + k1] != an) continue;
-                    #pragma unroll 1
                     for (int k2 = 0; k2 < 2 * nh; ++k2) {
-                        if (s_cmap[q * 8 + k2] != bn) continue;
-                        const double v = s_cN[q * 4 + k1 % nh] * s_cN[q * 4 + k2 % nh] * s_cRDR[q * DIM * DIM + da * DIM + db];
+                        const int bn = s_cmap[q * 8 + k2];
                         const double sgn = ((k1 < nh) == (k2 < nh)) ? 1.0 : -1.0;
-                        acc += s_cw[q] * (sgn * v);
+                        for (int db = 0; db < DIM; ++db) {
+                            const int j = bn * DIM + db;
+                            if (j % nsplit != grp) continue;
+                            const double v = s_cN[q * 4 + k1 % nh] * s_cN[q * 4 + k2 % nh] * s_cRDR[q * DIM * DIM + da * DIM + db];
+                            Ki[j] += s_cw[q] * (sgn * v);
+                        }
                     }
                 }
             }
-            s_K[e] = acc;
         }
+    }
     if (DO_F)
         for (int i = tid; i < M; i += CUT_THREADS) {
             const int an = i / DIM, d = i % DIM;
