@@ -50,11 +50,23 @@ def main():
     ap.add_argument("--n-incl", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-learn", action="store_true", help="skip the learned-guess load steps")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
+    ap.add_argument("--c2-n", type=int, default=128)
+    ap.add_argument("--c2-incl", type=int, default=200)
+    ap.add_argument("--c2-steps", type=int, default=3, help="load increments timed (of 20 to 2%% strain)")
+    ap.add_argument("--c3-rves", type=int, default=256)
+    ap.add_argument("--c3-nmin", type=int, default=64)
+    ap.add_argument("--c3-nmax", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c2":
+        return bench_c2(args)
+    if args.workload == "c3":
+        return bench_c3(args)
 
     from pigfem_assignment_amd import configs, _lib as L
     from pigfem_assignment_amd.api import DeviceContext
@@ -118,6 +130,9 @@ def main():
                                                                            si.pcg_iterations, si.res / si.res0),
                   flush=True)
 
+    if not args.no_learn:
+        extra.update(learned_vs_plain(n=64, n_incl=40, steps=3, log=lambda m: print(m, flush=True)))
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -153,6 +168,123 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
         "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, train_steps=5, epochs=30, log=print,
+                     ctx=None):
+    """Time-to-1e-8 residual per load step of a 3D cohesive inclusion problem:
+    plain FP64 PCG from a zero start vs the learned start with the FP32 inner
+    solve (FP64 residual), and the learned start with FP64.  The network is
+    trained first on plain solves of two smaller problems (its voxel grid does
+    not depend on the mesh)."""
+    from pigfem_assignment_amd import learn, configs
+    from pigfem_assignment_amd.api import GuessNet
+    t0 = time.time()
+    net = GuessNet(seed=3)
+    train = [learn.cohesive_problem(train_n, train_incl, seed) for seed in (501, 502)]
+    loss = learn.train_net(net, train, train_steps, 0.05, epochs=epochs, lr=1e-3, log=log)
+    t_train = time.time() - t0
+    if prob is None:
+        prob = (configs.c2_problem(n=n, n_incl=n_incl), configs.c2_inclusions(n_incl, n))
+    dfrac = 0.05  # 20 increments to 2 % strain
+    out = {"learn_train_s": t_train, "learn_loss_first": float(loss[0]), "learn_loss_last": float(loss[-1]),
+           "learn_workload": "unit cube %d^3 x 6 TET4, %d spheres, bilinear cohesive, first %d of 20 increments" % (
+               n, n_incl, steps)}
+    modes = [("plain_fp64", 0, 64, 1e-10), ("learned_fp32", 2, 32, 1e-5), ("learned_fp64", 2, 64, 1e-10)]
+    flags = {}
+    for name, ug, prec, lin in modes:
+        recs, fl, ctx = learn.run_steps(prob[0], prob[1], steps, dfrac, use_guess=ug, net=net, lin_rtol=lin,
+                                        precision=prec, ctx=ctx)
+        flags[name] = fl
+        out["%s_ms_per_step" % name] = [r["ms"] for r in recs]
+        out["%s_pcg_per_step" % name] = [r["pcg"] for r in recs]
+        out["%s_newton_per_step" % name] = [r["newton"] for r in recs]
+        log("# %s: ms per step %s, PCG %s, Newton %s" % (name, ["%.1f" % r["ms"] for r in recs],
+                                                         [r["pcg"] for r in recs], [r["newton"] for r in recs]))
+    same = all(np.array_equal(a, b) for m in flags for a, b in zip(flags["plain_fp64"], flags[m]))
+    out["damage_flags_identical"] = bool(same)
+    out["speedup_learned_fp32_vs_plain"] = sum(out["plain_fp64_ms_per_step"]) / sum(out["learned_fp32_ms_per_step"])
+    log("# learned+FP32 vs plain FP64: %.2fx, damage flags identical: %s" % (out["speedup_learned_fp32_vs_plain"], same))
+    return out
+
+
+def bench_c2(args):
+    from pigfem_assignment_amd import configs, _lib as L
+    from pigfem_assignment_amd.api import DeviceContext
+    t0 = time.time()
+    prob = configs.c2_problem(n=args.c2_n, n_incl=args.c2_incl)
+    incl = configs.c2_inclusions(args.c2_incl, args.c2_n)
+    info = prob.info
+    print("# C2 setup: %d tets (%d cut), %d dofs, %d cohesive points, host %.1f s" % (
+        info.n_elem, info.n_cut, info.n_free, info.n_cpt, time.time() - t0), flush=True)
+    ctx = DeviceContext(prob)
+    flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
+    for _ in range(args.warmup):
+        ctx.assemble(flags, dt=0.05)
+    ms = ctx.time_assemble(flags, reps=args.steps)
+    ms_call, ms_cut, ms_gather = ctx.time_assemble_split(flags, reps=args.steps)
+    nnz_free_rows = int(ctx.pattern()[0][info.n_free])
+    bytes_gather = gather_bytes(3, nnz_free_rows, info.n_free, info.n_all_nodes, info.n_elem)
+    achieved = bytes_gather / (ms_gather * 1e-3) / 1e9
+    print("# C2 assembly K+f: %.2f ms (cut pass %.2f, gather %.2f), %.1f GB/s" % (ms, ms_cut, ms_gather, achieved),
+          flush=True)
+    extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather}
+    extra.update(learned_vs_plain(args.c2_n, args.c2_incl, args.c2_steps, prob=(prob, incl),
+                                  log=lambda m: print(m, flush=True), ctx=ctx))
+    line = {
+        "metric": "element assemblies/s; time-to-1e-8 residual per load step, learned vs plain",
+        "value": info.n_elem / (ms * 1e-3), "unit": "element assemblies/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D unit cube %d^3 x 6 TET4, %d RSA spheres r~U[0.03,0.06], bilinear cohesive "
+                               "interfaces, K+f assembly and Newton load steps" % (args.c2_n, args.c2_incl),
+                   "elements": info.n_elem, "cut_elements": info.n_cut, "dofs": info.n_free},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+def bench_c3(args):
+    from pigfem_assignment_amd import batch, _lib as L
+    from pigfem_assignment_amd.api import DeviceContext
+    rng = np.random.RandomState(20261202)
+    sizes = rng.randint(args.c3_nmin, args.c3_nmax + 1, size=args.c3_rves)
+    t0 = time.time()
+    parts = [batch.rve_problem(int(N), 1000 + k)[0] for k, N in enumerate(sizes)]
+    B = batch.BatchProblem(parts)
+    print("# C3 set-up: %d RVEs, %d elements (%d cut), %d dofs, host %.1f s" % (
+        len(parts), B.n_elem, B.n_cut, B.n_free, time.time() - t0), flush=True)
+    ctx = DeviceContext(B)
+    flags = L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS
+    for _ in range(args.warmup):
+        ctx.assemble(flags)
+    ms_asm = ctx.time_assemble(flags, reps=args.steps)
+    UP = B.case_values()
+    RHS = ctx.rhs_prescribed(UP)
+    X, its, rel, ms_solve, tune = ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-8)
+    R = ctx.reactions(X, UP)
+    Ce = batch.effective_stiffness(B, R)
+    print("# C3 assembly %.2f ms (%.3e elem/s); batched PCG 3 cases x %d RVEs: %.1f ms, iterations %d..%d, "
+          "tuned (threads/row, rows/tile) %s" % (ms_asm, B.n_elem / (ms_asm * 1e-3), len(parts), ms_solve,
+                                                 its.min(), its.max(), tune), flush=True)
+    line = {
+        "metric": "element assemblies/s; time-to-1e-8 residual per load step, learned vs plain",
+        "value": B.n_elem / (ms_asm * 1e-3), "unit": "element assemblies/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_asm, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d 2D RVEs N~U[%d,%d] QUAD4, 20-40%% inclusions, 3 homogenisation load cases, "
+                               "one batched assembly + segmented multi-RHS PCG to 1e-8" % (
+                                   len(parts), args.c3_nmin, args.c3_nmax),
+                   "elements": B.n_elem, "dofs": B.n_free},
+        "roofline": None, "cpu_baseline": None,
+        "extra": {"batched_solve_ms": ms_solve, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
+                  "iterations_total": int(its.sum()), "autotuned": list(tune),
+                  "C_eff_mean": Ce.mean(axis=0).tolist()},
     }
     print(json.dumps(line))
 

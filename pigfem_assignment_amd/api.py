@@ -224,6 +224,18 @@ class DeviceContext:
     def isv_commit(self):
         self._chk(self.lib.pgf_isv_commit(self.h))
 
+    def set_isv(self, dmax_conv):
+        d = np.ascontiguousarray(dmax_conv, np.float64)
+        self._chk(self.lib.pgf_set_isv(self.h, L.ptr(d, C.c_double)))
+
+    def reset(self):
+        """Back to the unloaded state: zero displacement and damage."""
+        self.set_solution(np.zeros((self.prob.dim, self.prob.info.n_all_nodes)))
+        n = C.c_int64()
+        self.lib.pgf_n_cohesive_points(self.h, C.byref(n))
+        if n.value:
+            self.set_isv(np.zeros(n.value))
+
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.csr.n_free)

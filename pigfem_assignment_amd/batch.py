@@ -38,9 +38,13 @@ def rve_inclusions(N, seed, vf_lo=0.2, vf_hi=0.4, n_lo=6, n_hi=20):
         try:
             return configs.rsa(n, 0.85 * r, 1.15 * r, 2, gap=gap, seed=seed, max_tries=20_000)
         except RuntimeError:
-            vf *= 0.9  # too dense for this mesh: lower the fraction
-            if vf < 0.05:
-                raise
+            # too dense for this mesh: fewer inclusions, then a lower fraction
+            if n > 2:
+                n -= 1
+            else:
+                vf *= 0.9
+                if vf < 0.02:
+                    raise
 
 
 def rve_problem(N, seed, materials=(configs.MATRIX, configs.INCLUSION), inclusions=None):
@@ -101,24 +105,24 @@ class BatchProblem:
             newd = np.where(dk < d.n_free, dk + F_off[k], NF + P_off[k] + (dk - d.n_free)).astype(np.int32)
             dof[:, reg] = newd[:, : d.n_nodes]
             dof[:, enr] = newd[:, d.n_nodes:]
-            conn.append(_arr(d.conn, 4 * d.n_elem, np.int32) + R_off[k])
+            conn.append((_arr(d.conn, 4 * d.n_elem, np.int32) + R_off[k]).astype(np.int32))
             emat.append(_arr(d.elem_mat, d.n_elem, np.int32))
             nc = d.n_cut
-            cut_elem.append(_arr(d.cut_elem, nc, np.int32) + e_off)
+            cut_elem.append((_arr(d.cut_elem, nc, np.int32) + e_off).astype(np.int32))
             cut_ne.append(_arr(d.cut_n_enrich, nc, np.int32))
             ce = _arr(d.cut_enrich, 16 * nc, np.int32).reshape(max(nc, 0), 16)
             ne = cut_ne[-1]
             mask = np.arange(16)[None, :] < ne[:, None]
-            ce = np.where(mask, ce + E_off[k], ce)
+            ce = np.where(mask, ce + E_off[k], ce).astype(np.int32)
             cut_en.append(ce.ravel())
             vp = _arr(d.vpt_ptr, nc + 1, np.int64) if nc else np.zeros(1, np.int64)
             nv = int(vp[-1])
-            vpt_ptr.append(vp[1:] + v_off)
+            vpt_ptr.append((vp[1:] + v_off).astype(np.int64))
             vpt_w.append(_arr(d.vpt_w, nv, np.float64))
             vpt_mat.append(_arr(d.vpt_mat, nv, np.int32))
             go = _arr(d.vpt_grad_off, nv, np.int64)
             ng = int(p.info.n_vgrad)
-            vpt_goff.append(go + g_off)
+            vpt_goff.append((go + g_off).astype(np.int64))
             vpt_grad.append(_arr(d.vpt_grad, ng, np.float64))
             if p.info.n_cpt > 0:
                 raise ValueError("cohesive RVEs are not batched")

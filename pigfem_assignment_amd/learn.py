@@ -42,10 +42,14 @@ def cohesive_problem(n, n_incl, seed, law=configs.BILINEAR_C2, r_lo=0.06, r_hi=0
 
 
 def run_steps(prob, incl, n_steps, frac_step, use_guess=0, net=None, record=False, lin_rtol=1e-10, dt=None,
-              precision=64, callback=None):
-    """Solve load steps frac = k * frac_step, k = 1..n_steps; returns per-step records
-    and the damage flags after every step."""
-    ctx = DeviceContext(prob)
+              precision=64, callback=None, ctx=None):
+    """Solve load steps frac = k * frac_step, k = 1..n_steps from the unloaded
+    state; returns per-step records, the damage flags after every step and the
+    context (pass it back in to reuse its device set-up)."""
+    if ctx is None:
+        ctx = DeviceContext(prob)
+    else:
+        ctx.reset()
     ctx.learn_attach(net, indicator=indicator(incl))
     out, flags = [], []
     for k in range(1, n_steps + 1):

@@ -94,8 +94,9 @@ def test_learned_guess_reduces_pcg_work():
     assert loss[-1] < loss[0]
     prob, incl = learn.cohesive_problem(n, n_incl, 103)
     res = {}
+    ctx = None
     for mode in (0, 3, 2):
-        recs, flags, _ = learn.run_steps(prob, incl, steps, dfrac, use_guess=mode, net=net)
+        recs, flags, ctx = learn.run_steps(prob, incl, steps, dfrac, use_guess=mode, net=net, ctx=ctx)
         res[mode] = (recs, flags)
         print("mode %d: pcg per step %s, newton %s, ms %s" % (
             mode, [r["pcg"] for r in recs], [r["newton"] for r in recs], ["%.0f" % r["ms"] for r in recs]))
