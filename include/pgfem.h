@@ -131,8 +131,12 @@ typedef struct pgf_newton_opts {
     int32_t precision;     /* 64: FP64 K and PCG; 32: FP32 K and PCG, FP64 residual */
     double lin_rtol;       /* PCG relative tolerance per Newton iteration */
     int32_t lin_maxit;
-    int32_t use_guess;     /* 1: start PCG from the device guess buffer (pgf_set_guess) */
+    int32_t use_guess;     /* 1: start PCG from the device guess buffer (pgf_set_guess); 2, 3: learned guess
+                              (pgfem_learn.h) */
     double dt;             /* time increment passed to the cohesive law */
+    int32_t adaptive_lin;  /* 1: per-iteration PCG tolerance max(lin_rtol, 0.5 rel_tol ||r0|| / ||r_k||)
+                              (inexact Newton forcing term); 0: lin_rtol every iteration */
+    int32_t reserved;
 } pgf_newton_opts;
 
 typedef struct pgf_step_info {

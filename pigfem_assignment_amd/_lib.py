@@ -51,7 +51,7 @@ class SolveInfo(C.Structure):
 class NewtonOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("abs_tol", C.c_double), ("max_iter", C.c_int32),
                 ("precision", C.c_int32), ("lin_rtol", C.c_double), ("lin_maxit", C.c_int32),
-                ("use_guess", C.c_int32), ("dt", C.c_double)]
+                ("use_guess", C.c_int32), ("dt", C.c_double), ("adaptive_lin", C.c_int32), ("reserved", C.c_int32)]
 
 
 class StepInfo(C.Structure):

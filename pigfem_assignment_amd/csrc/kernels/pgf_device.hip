@@ -916,17 +916,21 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
             HIPCHK(hipEventElapsedTime(&ms, a0, a1));
             ms_asm += ms;
             pgf_solve_info si{};
+            // inexact Newton: solve only as far as this iteration needs to reach the target
+            double lin_tol = o->lin_rtol;
+            if (o->adaptive_lin && res > 0.0)
+                lin_tol = std::min(0.5, std::max(o->lin_rtol, 0.5 * o->rel_tol * res0 / res));
             bool g = o->use_guess == 1 && c->have_guess && it == 0;
             if (it == 0 && c->learn) g = pgfd::learn_step_start(*c, o->use_guess, fp32) || g;
             if (fp32) {
                 if (g) hipLaunchKernelGGL(pgfd::k_to_f32, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x32);
                 pgfd::run_pcg<float>(*c, c->K32, c->r, true, c->x32, c->r32, c->z32, c->p032, c->p132, c->q32, c->dinv32, g,
-                                     o->lin_rtol, o->lin_maxit, &si);
+                                     lin_tol, o->lin_maxit, &si);
                 hipLaunchKernelGGL((pgfd::k_update_U<float>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
                                    c->free_slot, c->x32, c->U);
             } else {
                 if (g) hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x);
-                pgfd::run_pcg<double>(*c, c->K, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g, o->lin_rtol,
+                pgfd::run_pcg<double>(*c, c->K, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g, lin_tol,
                                       o->lin_maxit, &si);
                 hipLaunchKernelGGL((pgfd::k_update_U<double>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
                                    c->free_slot, c->x, c->U);

@@ -2,11 +2,14 @@
 
 Forward outputs and parameter gradients of the bf16-MFMA kernels are compared
 with the oracle's float64 evaluation at the same bf16 rounding points.  The
-remaining differences come from FP32 accumulation order and occasional
-one-ulp bf16 flips of activations, so the bar is normwise: ||gpu - oracle|| /
-||oracle|| <= 1e-2 for the output and for each layer's weight and bias
-gradients (the oracle itself is pinned by finite differences in
-tests/test_cpu_cnn_oracle.py).
+remaining differences come from FP32 accumulation order, which moves values
+across bf16 rounding boundaries (one-ulp flips, 2^-8 relative) and ReLU
+masks, compounding over the layers.  The bars are normwise: output
+||gpu - oracle|| / ||oracle|| <= 1e-2; every layer's weight and bias gradient
+within 5e-2 of the oracle's and at most 0.35 times the distance between the
+oracle with and without bf16 rounding (so the device computes the rounded
+network, not merely a network near it).  The oracle itself is pinned by
+finite differences in tests/test_cpu_cnn_oracle.py.
 """
 import numpy as np
 import pytest
@@ -54,12 +57,16 @@ def test_gradients_match_oracle():
     g = net.backward(x, dy)
     dy16 = np.zeros((G, G, G, 16))
     dy16[..., :3] = dy
-    go = N.backward(net.params().astype(np.float64), x, dy16)
+    p = net.params().astype(np.float64)
+    go = N.backward(p, x, dy16)
+    gf = N.backward(p, x, dy16, rounding=False)
     for l, (sw, sb) in enumerate(_layer_slices()):
         ew = np.linalg.norm(g[sw] - go[sw]) / np.linalg.norm(go[sw])
         eb = np.linalg.norm(g[sb] - go[sb]) / max(np.linalg.norm(go[sb]), 1e-30)
-        print("layer %d: weight grad rel err %.3e, bias grad rel err %.3e" % (l, ew, eb))
-        assert ew <= TOL and eb <= TOL
+        rw = np.linalg.norm(gf[sw] - go[sw]) / np.linalg.norm(go[sw])
+        print("layer %d: weight grad rel err %.3e (bf16 rounding effect %.3e), bias grad rel err %.3e" % (l, ew, rw, eb))
+        assert ew <= 5e-2 and eb <= 5e-2
+        assert ew <= 0.35 * rw
 
 
 def test_training_reduces_loss():
