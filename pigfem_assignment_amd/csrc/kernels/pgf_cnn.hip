@@ -912,7 +912,7 @@ struct LearnState {
     float* tgt = nullptr;      // [NVOX][3]
     int32_t* node_vox = nullptr;
     int32_t* cpt_vox = nullptr;
-    double *U_prev = nullptr, *U_prevprev = nullptr, *U_start = nullptr;
+    double *U_prev = nullptr, *U_prevprev = nullptr, *U_start = nullptr, *U_iter = nullptr;
     int n_commits = 0;
     bool have_features = false;
     double* Z = nullptr;       // [4][n_free]
@@ -927,8 +927,8 @@ constexpr int DOT_BLOCKS = 512;
 
 void learn_free(LearnState* s) {
     if (!s) return;
-    void* p[] = {s->ind, s->feat, s->cnt, s->tgt, s->node_vox, s->cpt_vox, s->U_prev, s->U_prevprev, s->U_start, s->Z, s->KZ,
-                 s->part};
+    void* p[] = {s->ind, s->feat, s->cnt, s->tgt, s->node_vox, s->cpt_vox, s->U_prev, s->U_prevprev, s->U_start, s->U_iter,
+                 s->Z, s->KZ, s->part};
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (s->h_part) (void)hipHostFree(s->h_part);
@@ -949,8 +949,10 @@ static void normalise(Ctx& c, LearnState& L, float* x, int ld, int ch, int nc) {
         hipLaunchKernelGGL(pgfn::k_scale_ch, dim3(gfor(pgfn::NVOX)), dim3(256), 0, c.stream, x, ld, ch, nc, (float)(1.0 / rms));
 }
 
-// features at the start of a load step: indicator, damage, residual, previous increment
-static void build_features(Ctx& c, LearnState& L) {
+// features at the start of a load step (or of a later Newton iteration: the
+// residual channels then hold that iteration's residual): indicator, damage,
+// residual, previous increment
+static void build_features(Ctx& c, LearnState& L, bool step_start) {
     using namespace pgfn;
     hipLaunchKernelGGL(k_feat_init, dim3(gfor(NVOX)), dim3(256), 0, c.stream, (const float*)L.ind, L.feat, L.cnt);
     if (c.n_cpt > 0)
@@ -966,19 +968,33 @@ static void build_features(Ctx& c, LearnState& L) {
     NCHK(hipGetLastError());
     normalise(c, L, L.feat, IN_CH, 2, 3);
     normalise(c, L, L.feat, IN_CH, 5, 3);
-    NCHK(hipMemcpyAsync(L.U_start, c.U, sizeof(double) * c.dim * c.n_all, hipMemcpyDeviceToDevice, c.stream));
-    L.have_features = true;
+    if (step_start) {
+        NCHK(hipMemcpyAsync(L.U_start, c.U, sizeof(double) * c.dim * c.n_all, hipMemcpyDeviceToDevice, c.stream));
+        L.have_features = true;
+    }
 }
 
-bool learn_step_start(Ctx& c, int use_guess, bool fp32) {
+bool learn_guess(Ctx& c, int use_guess, bool fp32, int it) {
     using namespace pgfn;
     LearnState& L = *c.learn;
     hipEvent_t e0, e1;
     NCHK(hipEventCreate(&e0));
     NCHK(hipEventCreate(&e1));
     NCHK(hipEventRecord(e0, c.stream));
-    build_features(c, L);
-    if (use_guess != 2 && use_guess != 3) {
+    const bool guess = use_guess == 2 || use_guess == 3;
+    if (it == 0 || (guess && use_guess == 2)) build_features(c, L, it == 0);
+    // column 3: the previous load increment at iteration 0, the previous Newton
+    // correction U - U_iter afterwards
+    const double* c3a = nullptr;
+    const double* c3b = nullptr;
+    if (it == 0) {
+        if (L.n_commits >= 2) { c3a = L.U_prev; c3b = L.U_prevprev; }
+    } else {
+        c3a = c.U;
+        c3b = L.U_iter;
+    }
+    if (!guess) {
+        NCHK(hipMemcpyAsync(L.U_iter, c.U, sizeof(double) * c.dim * c.n_all, hipMemcpyDeviceToDevice, c.stream));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return false;
@@ -993,12 +1009,12 @@ bool learn_step_start(Ctx& c, int use_guess, bool fp32) {
         forward(L.net, L.net->x);
         L.net->stream = keep;
     }
-    const bool have_prev = L.n_commits >= 2;
+    const bool have_prev = c3a != nullptr;
     const int64_t nf = c.n_free;
     hipLaunchKernelGGL(k_prolong, dim3(gfor(nf)), dim3(256), 0, c.stream, nf, c.n_all, (const int64_t*)c.free_slot,
-                       (const double*)c.X, L.box, (const float*)(use_net ? L.net->y : nullptr), 16, use_net ? 1 : 0,
-                       (const double*)(have_prev ? L.U_prev : nullptr), (const double*)(have_prev ? L.U_prevprev : nullptr),
+                       (const double*)c.X, L.box, (const float*)(use_net ? L.net->y : nullptr), 16, use_net ? 1 : 0, c3a, c3b,
                        L.Z);
+    NCHK(hipMemcpyAsync(L.U_iter, c.U, sizeof(double) * c.dim * c.n_all, hipMemcpyDeviceToDevice, c.stream));
     for (int j = 0; j < 4; ++j) {
         if (fp32)
             hipLaunchKernelGGL(k_spmv_col<float>, dim3(gfor(nf)), dim3(256), 0, c.stream, nf, (const int32_t*)c.rowptr,
@@ -1141,6 +1157,7 @@ PGF_API int pgf_learn_attach(pgf_ctx* h, pgf_net* net, const double* box, const 
         L->U_prev = nalloc<double>((size_t)c->dim * c->n_all);
         L->U_prevprev = nalloc<double>((size_t)c->dim * c->n_all);
         L->U_start = nalloc<double>((size_t)c->dim * c->n_all);
+        L->U_iter = nalloc<double>((size_t)c->dim * c->n_all);
         L->Z = nalloc<double>((size_t)4 * c->n_free);
         L->KZ = nalloc<double>((size_t)4 * c->n_free);
         L->part = nalloc<double>((size_t)pgfn::NDOT * pgfd::DOT_BLOCKS);

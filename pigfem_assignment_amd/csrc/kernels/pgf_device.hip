@@ -921,7 +921,8 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
             if (o->adaptive_lin && res > 0.0)
                 lin_tol = std::min(0.5, std::max(o->lin_rtol, 0.5 * o->rel_tol * res0 / res));
             bool g = o->use_guess == 1 && c->have_guess && it == 0;
-            if (it == 0 && c->learn) g = pgfd::learn_step_start(*c, o->use_guess, fp32) || g;
+            if (c->learn && (it == 0 || o->use_guess == 2 || o->use_guess == 3))
+                g = pgfd::learn_guess(*c, o->use_guess, fp32, it) || g;
             if (fp32) {
                 if (g) hipLaunchKernelGGL(pgfd::k_to_f32, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x32);
                 pgfd::run_pcg<float>(*c, c->K32, c->r, true, c->x32, c->r32, c->z32, c->p032, c->p132, c->q32, c->dinv32, g,

@@ -81,12 +81,13 @@ template <typename T>
 void fill_async(Ctx& c, T* p, int64_t n, T v);
 void enqueue_assembly(Ctx& c, double dt, uint32_t flags);
 
-// Learned initial guess (pgf_cnn.hip).  Called once per Newton step after the
-// first Jacobian assembly, with the step residual in c.r: records the
-// step-start features and U; for use_guess 2 (network + previous increment)
-// or 3 (previous increment only) writes a Galerkin-scaled guess to c.guess
-// and returns true.
-bool learn_step_start(Ctx& c, int use_guess, bool fp32);
+// Learned initial guess (pgf_cnn.hip).  Called after the Jacobian assembly of
+// Newton iteration `it` with that iteration's residual in c.r.  Iteration 0
+// records the step-start features and U (training samples).  For use_guess 2
+// (network + previous increment / previous Newton correction) or 3 (previous
+// increment / correction only) it writes the Galerkin-scaled guess of this
+// iteration's solve to c.guess and returns true.
+bool learn_guess(Ctx& c, int use_guess, bool fp32, int it);
 void learn_free(LearnState* s);
 Ctx* ctx_of(pgf_ctx* h);
 
