@@ -116,32 +116,31 @@ __device__ inline double force_entry(const double* g, int d, const double* s) {
     return g[2] * s[2] + g[1] * s[3] + g[0] * s[4];
 }
 
-// Geometry of an uncut parent element: QUAD4 2x2 Gauss points (w = J*1) or
-// the linear tet's single merged point (the five-point rule's weights
-// -0.8 + 4*0.45 applied to the common B), Elem::ShapeFunctions at each.
+// Geometry of an uncut parent element at its quadrature point q: QUAD4 2x2
+// Gauss points (w = J*1) or the linear tet's single merged point (the
+// five-point rule's weights -0.8 + 4*0.45 applied to the common B),
+// Elem::ShapeFunctions at the point.
 template <int DIM>
-__device__ inline void uncut_geometry(const double (*X)[DIM], double tet_scale, double (*g)[4 * DIM], double* w) {
+__device__ inline void qp_geometry(const double (*X)[DIM], double tet_scale, int q, double* g, double& w) {
     if (DIM == 2) {
         const double r0 = 0.57735026918962576450914878050196;
-        for (int q = 0; q < 4; ++q) {
-            const double r = (q % 2 == 0) ? -r0 : r0, s = (q / 2 == 0) ? -r0 : r0;
-            const double dr[4][2] = {{-0.25 * (1.0 - s), -0.25 * (1.0 - r)},
-                                     {0.25 * (1.0 - s), -0.25 * (1.0 + r)},
-                                     {0.25 * (1.0 + s), 0.25 * (1.0 + r)},
-                                     {-0.25 * (1.0 + s), 0.25 * (1.0 - r)}};
-            double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
-            for (int n = 0; n < 4; ++n) {
-                J00 += dr[n][0] * X[n][0]; J01 += dr[n][0] * X[n][1];
-                J10 += dr[n][1] * X[n][0]; J11 += dr[n][1] * X[n][1];
-            }
-            const double det = J00 * J11 - J01 * J10;
-            const double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
-            for (int n = 0; n < 4; ++n) {
-                g[q][n * 2 + 0] = i00 * dr[n][0] + i01 * dr[n][1];
-                g[q][n * 2 + 1] = i10 * dr[n][0] + i11 * dr[n][1];
-            }
-            w[q] = det * 1.0;
+        const double r = (q % 2 == 0) ? -r0 : r0, s = (q / 2 == 0) ? -r0 : r0;
+        const double dr[4][2] = {{-0.25 * (1.0 - s), -0.25 * (1.0 - r)},
+                                 {0.25 * (1.0 - s), -0.25 * (1.0 + r)},
+                                 {0.25 * (1.0 + s), 0.25 * (1.0 + r)},
+                                 {-0.25 * (1.0 + s), 0.25 * (1.0 - r)}};
+        double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+        for (int n = 0; n < 4; ++n) {
+            J00 += dr[n][0] * X[n][0]; J01 += dr[n][0] * X[n][1];
+            J10 += dr[n][1] * X[n][0]; J11 += dr[n][1] * X[n][1];
         }
+        const double det = J00 * J11 - J01 * J10;
+        const double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
+        for (int n = 0; n < 4; ++n) {
+            g[n * 2 + 0] = i00 * dr[n][0] + i01 * dr[n][1];
+            g[n * 2 + 1] = i10 * dr[n][0] + i11 * dr[n][1];
+        }
+        w = det * 1.0;
     } else {
         double J[3][3];
         for (int i = 0; i < 3; ++i) {
@@ -164,11 +163,16 @@ __device__ inline void uncut_geometry(const double (*X)[DIM], double tet_scale, 
         inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
         const double dr[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
         for (int n = 0; n < 4; ++n)
-            for (int i = 0; i < 3; ++i) g[0][n * 3 + i] = inv[i][0] * dr[n][0] + inv[i][1] * dr[n][1] + inv[i][2] * dr[n][2];
+            for (int i = 0; i < 3; ++i) g[n * 3 + i] = inv[i][0] * dr[n][0] + inv[i][1] * dr[n][1] + inv[i][2] * dr[n][2];
         double ws = det * -0.8;
-        for (int q = 0; q < 4; ++q) ws += det * 0.45;
-        w[0] = ws * tet_scale;
+        for (int k = 0; k < 4; ++k) ws += det * 0.45;
+        w = ws * tet_scale;
     }
+}
+template <int DIM>
+__device__ inline void uncut_geometry(const double (*X)[DIM], double tet_scale, double (*g)[4 * DIM], double* w) {
+    constexpr int NQ = DIM == 2 ? 4 : 1;
+    for (int q = 0; q < NQ; ++q) qp_geometry<DIM>(X, tet_scale, q, g[q], w[q]);
 }
 
 template <typename KT>
@@ -500,29 +504,37 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                 const int mat = a.el_mat[e];
                 double D[S * S];
                 for (int q = 0; q < S * S; ++q) D[q] = a.Dmat[mat * 36 + q];
-                double g[NQ][4 * DIM], w[NQ];
-                uncut_geometry<DIM>(X, a.tet_scale, g, w);
-                if (DO_F) {
-                    double u[4 * DIM];
+                double u[4 * DIM];
+                if (DO_F)
                     for (int q = 0; q < 4; ++q)
                         for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
-                    for (int q = 0; q < NQ; ++q) {
+                double Kr[4 * DIM];
+                for (int j = 0; j < 4 * DIM; ++j) Kr[j] = 0.0;
+                // one quadrature point at a time keeps the register footprint small
+#pragma unroll 1
+                for (int q = 0; q < NQ; ++q) {
+                    double g[4 * DIM], w;
+                    qp_geometry<DIM>(X, a.tet_scale, q, g, w);
+                    // the row node's gradient, selected without indexing the
+                    // register array by the run-time slot
+                    double gr[DIM];
+                    for (int i = 0; i < DIM; ++i)
+                        gr[i] = la == 0 ? g[i] : la == 1 ? g[DIM + i] : la == 2 ? g[2 * DIM + i] : g[3 * DIM + i];
+                    if (DO_F) {
                         double eps[S], sig[S];
-                        strain_of<DIM>(4, g[q], u, eps);
+                        strain_of<DIM>(4, g, u, eps);
                         stress_of<DIM>(D, eps, sig);
-                        Pa += w[q] * force_entry<DIM>(&g[q][la * DIM], da, sig);
+                        Pa += w * force_entry<DIM>(gr, da, sig);
+                    }
+                    if (DO_K) {
+                        double parts[S];
+                        k_parts<DIM>(gr, da, D, parts);
+                        for (int bn = 0; bn < 4; ++bn)
+                            for (int db = 0; db < DIM; ++db) Kr[bn * DIM + db] += w * k_entry<DIM>(parts, &g[bn * DIM], db);
                     }
                 }
                 if (DO_K) {
                     const uint64_t off = a.inc_off[k];
-                    double Kr[4 * DIM];
-                    for (int j = 0; j < 4 * DIM; ++j) Kr[j] = 0.0;
-                    for (int q = 0; q < NQ; ++q) {
-                        double parts[S];
-                        k_parts<DIM>(&g[q][la * DIM], da, D, parts);
-                        for (int bn = 0; bn < 4; ++bn)
-                            for (int db = 0; db < DIM; ++db) Kr[bn * DIM + db] += w[q] * k_entry<DIM>(parts, &g[q][bn * DIM], db);
-                    }
                     for (int bn = 0; bn < 4; ++bn) {
                         const int of = (int)((off >> (16 * bn)) & 0xff), op = (int)((off >> (16 * bn + 8)) & 0xff);
                         int cf = 0, cp = 0;
