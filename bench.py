@@ -54,7 +54,9 @@ def main():
     ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
     ap.add_argument("--c2-n", type=int, default=128)
     ap.add_argument("--c2-incl", type=int, default=200)
-    ap.add_argument("--c2-steps", type=int, default=3, help="load increments timed (of 20 to 2%% strain)")
+    ap.add_argument("--c2-steps", type=int, default=2, help="load increments timed (of 20 to 2%% strain)")
+    ap.add_argument("--c2-modes", default="plain_fp64,learned_fp32_forcing",
+                    help="comma list of " + ",".join(ALL_MODES))
     ap.add_argument("--c3-rves", type=int, default=256)
     ap.add_argument("--c3-nmin", type=int, default=64)
     ap.add_argument("--c3-nmax", type=int, default=512)
@@ -172,8 +174,12 @@ def main():
     print(json.dumps(line))
 
 
+ALL_MODES = {"plain_fp64": (0, 64, 1e-10, False), "plain_fp64_forcing": (0, 64, 1e-10, True),
+             "learned_fp64_forcing": (2, 64, 1e-10, True), "learned_fp32_forcing": (2, 32, 1e-5, True)}
+
+
 def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, train_steps=5, epochs=30, log=print,
-                     ctx=None):
+                     ctx=None, modes=tuple(ALL_MODES)):
     """Time-to-1e-8 residual per load step of a 3D cohesive inclusion problem:
     plain FP64 PCG from a zero start vs the learned start with the FP32 inner
     solve (FP64 residual), and the learned start with FP64.  The network is
@@ -194,8 +200,7 @@ def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, tra
                n, n_incl, steps)}
     # plain: zero start, PCG to 1e-10 in every Newton iteration; the others use
     # the inexact-Newton forcing term (each solve only as tight as needed)
-    modes = [("plain_fp64", 0, 64, 1e-10, False), ("plain_fp64_forcing", 0, 64, 1e-10, True),
-             ("learned_fp64_forcing", 2, 64, 1e-10, True), ("learned_fp32_forcing", 2, 32, 1e-5, True)]
+    modes = [(m,) + ALL_MODES[m] for m in modes]
     flags = {}
     for name, ug, prec, lin, adapt in modes:
         recs, fl, ctx = learn.run_steps(prob[0], prob[1], steps, dfrac, use_guess=ug, net=net, lin_rtol=lin,
@@ -206,15 +211,16 @@ def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, tra
         out["%s_newton_per_step" % name] = [r["newton"] for r in recs]
         log("# %s: ms per step %s, PCG %s, Newton %s" % (name, ["%.1f" % r["ms"] for r in recs],
                                                          [r["pcg"] for r in recs], [r["newton"] for r in recs]))
-    same = all(np.array_equal(a, b) for m in flags for a, b in zip(flags["plain_fp64"], flags[m]))
+    ref = flags[modes[0][0]]
+    same = all(np.array_equal(a, b) for m in flags for a, b in zip(ref, flags[m]))
     out["damage_flags_identical"] = bool(same)
     tot = {m[0]: sum(out["%s_ms_per_step" % m[0]]) for m in modes}
-    out["speedup_learned_fp32_vs_plain"] = tot["plain_fp64"] / tot["learned_fp32_forcing"]
-    out["speedup_learned_fp32_vs_plain_forcing"] = tot["plain_fp64_forcing"] / tot["learned_fp32_forcing"]
-    out["speedup_learned_fp64_vs_plain_forcing"] = tot["plain_fp64_forcing"] / tot["learned_fp64_forcing"]
-    log("# learned+FP32 vs plain FP64: %.2fx (vs plain FP64 with forcing %.2fx; learned FP64 vs plain forcing %.2fx), "
-        "damage flags identical: %s" % (out["speedup_learned_fp32_vs_plain"], out["speedup_learned_fp32_vs_plain_forcing"],
-                                        out["speedup_learned_fp64_vs_plain_forcing"], same))
+    for a, b in (("plain_fp64", "learned_fp32_forcing"), ("plain_fp64_forcing", "learned_fp32_forcing"),
+                 ("plain_fp64_forcing", "learned_fp64_forcing")):
+        if a in tot and b in tot:
+            out["speedup_%s_vs_%s" % (b, a)] = tot[a] / tot[b]
+            log("# %s vs %s: %.2fx" % (b, a, tot[a] / tot[b]))
+    log("# damage flags identical across modes: %s" % same)
     return out
 
 
@@ -240,7 +246,7 @@ def bench_c2(args):
           flush=True)
     extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather}
     extra.update(learned_vs_plain(args.c2_n, args.c2_incl, args.c2_steps, prob=(prob, incl),
-                                  log=lambda m: print(m, flush=True), ctx=ctx))
+                                  log=lambda m: print(m, flush=True), ctx=ctx, modes=args.c2_modes.split(",")))
     line = {
         "metric": "element assemblies/s; time-to-1e-8 residual per load step, learned vs plain",
         "value": info.n_elem / (ms * 1e-3), "unit": "element assemblies/s", "n_gpus": 1, "steps": args.steps,

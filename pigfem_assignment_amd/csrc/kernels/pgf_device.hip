@@ -299,6 +299,10 @@ static void enqueue_spmv(Ctx& c, const T* A, const T* x, T* y) {
 
 // Enqueue a whole PCG solve in chunks; the device flag stops the recurrence,
 // the host only polls the pinned scalars between chunks.
+static double wall_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 template <typename T>
 static int run_pcg(Ctx& c, const T* A, const void* b, bool b_is_double, T* x, T* r, T* z, T* p0, T* p1, T* q,
                    T* dinv, bool x0, double rtol, int maxit, pgf_solve_info* info) {
@@ -325,6 +329,9 @@ static int run_pcg(Ctx& c, const T* A, const void* b, bool b_is_double, T* x, T*
     const int chunk = 32;
     int it = 0;
     bool stop = false;
+    const char* pe = std::getenv("PGF_PROGRESS");
+    const double progress_s = pe ? std::atof(pe) : 0.0;
+    double last_report = wall_s();
     while (!stop) {
         for (int k = 0; k < chunk; ++k, ++it) {
             T* pold = (it % 2 == 0) ? p0 : p1;
@@ -344,6 +351,16 @@ static int run_pcg(Ctx& c, const T* A, const void* b, bool b_is_double, T* x, T*
         int dn;
         std::memcpy(&dn, c.h_scal + S_NSCAL, sizeof(int));
         if (dn || it >= maxit) stop = true;
+        // optional progress report of long solves (PGF_PROGRESS = seconds between lines)
+        if (progress_s > 0.0) {
+            const double now = wall_s();
+            if (now - last_report >= progress_s) {
+                last_report = now;
+                std::fprintf(stderr, "[pcg] %d iterations, ||r|| / ||r0|| = %.3e\n", (int)c.h_scal[S_ITER],
+                             c.h_scal[S_RR0] > 0 ? std::sqrt(c.h_scal[S_RR] / c.h_scal[S_RR0]) : 0.0);
+                std::fflush(stderr);
+            }
+        }
     }
     HIPCHK(hipEventRecord(e1, c.stream));
     HIPCHK(hipEventSynchronize(e1));
