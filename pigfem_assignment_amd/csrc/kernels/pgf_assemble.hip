@@ -135,7 +135,8 @@ __device__ inline void qp_geometry(const double (*X)[DIM], double tet_scale, int
             J10 += dr[n][1] * X[n][0]; J11 += dr[n][1] * X[n][1];
         }
         const double det = J00 * J11 - J01 * J10;
-        const double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
+        const double rd = 1.0 / det;  // one division per point
+        const double i00 = J11 * rd, i01 = -J01 * rd, i10 = -J10 * rd, i11 = J00 * rd;
         for (int n = 0; n < 4; ++n) {
             g[n * 2 + 0] = i00 * dr[n][0] + i01 * dr[n][1];
             g[n * 2 + 1] = i10 * dr[n][0] + i11 * dr[n][1];
@@ -151,16 +152,17 @@ __device__ inline void qp_geometry(const double (*X)[DIM], double tet_scale, int
         const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        const double rd = 1.0 / det;  // one division per element
         double inv[3][3];
-        inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
-        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
-        inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
-        inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
-        inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
-        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
-        inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
-        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
-        inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * rd;
+        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * rd;
+        inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * rd;
+        inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * rd;
+        inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * rd;
+        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * rd;
+        inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * rd;
+        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * rd;
+        inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * rd;
         const double dr[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
         for (int n = 0; n < 4; ++n)
             for (int i = 0; i < 3; ++i) g[n * 3 + i] = inv[i][0] * dr[n][0] + inv[i][1] * dr[n][1] + inv[i][2] * dr[n][2];
