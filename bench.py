@@ -174,8 +174,11 @@ def main():
     print(json.dumps(line))
 
 
+# (use_guess, precision, PCG tolerance floor, forcing term).  FP32 inner solves
+# stop at 1e-3 relative: far above where FP32 CG stalls on these matrices, the
+# FP64 Newton residual does the rest.
 ALL_MODES = {"plain_fp64": (0, 64, 1e-10, False), "plain_fp64_forcing": (0, 64, 1e-10, True),
-             "learned_fp64_forcing": (2, 64, 1e-10, True), "learned_fp32_forcing": (2, 32, 1e-5, True)}
+             "learned_fp64_forcing": (2, 64, 1e-10, True), "learned_fp32_forcing": (2, 32, 1e-3, True)}
 
 
 def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, train_steps=5, epochs=30, log=print,
