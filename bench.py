@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-learn", action="store_true", help="skip the learned-guess load steps")
     ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device-to-device five times (calibrates the HBM byte counters)")
     ap.add_argument("--c2-n", type=int, default=128)
     ap.add_argument("--c2-incl", type=int, default=200)
     ap.add_argument("--c2-steps", type=int, default=2, help="load increments timed (of 20 to 2%% strain)")
@@ -65,6 +67,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        return copy_probe()
     if args.workload == "c2":
         return bench_c2(args)
     if args.workload == "c3":
@@ -167,7 +171,7 @@ def main():
                    "elements": info.n_elem, "cut_elements": info.n_cut, "dofs": info.n_free,
                    "nnz": ctx.csr.nnz, "colors": ctx.csr.n_colors},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic()},
         "cpu_baseline": cpu,
         "extra": extra,
     }
@@ -179,6 +183,31 @@ def main():
 # FP64 Newton residual does the rest.
 ALL_MODES = {"plain_fp64": (0, 64, 1e-10, False), "plain_fp64_forcing": (0, 64, 1e-10, True),
              "learned_fp64_forcing": (2, 64, 1e-10, True), "learned_fp32_forcing": (2, 32, 1e-3, True)}
+
+
+def copy_probe():
+    """Plain device copy of a known size, for FETCH_SIZE / WRITE_SIZE calibration."""
+    import torch
+    n = 64 * 1024 * 1024  # float32 -> 256 MiB
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("copied 5 x %d bytes" % (4 * n), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the gather pass from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    f = os.path.join(ROOT, "profiles", "assembly_c1_hbm_traffic.txt")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        line = line.strip()
+        if line.startswith("traffic ") and "MB per launch" in line:
+            return float(line.split()[1]) * 1e6
+    return None
 
 
 def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, train_steps=5, epochs=30, log=print,

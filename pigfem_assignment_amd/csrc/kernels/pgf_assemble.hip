@@ -437,7 +437,9 @@ __global__ __launch_bounds__(CUT_THREADS) void k_cut(AsmArgs a, int64_t begin, i
 template <int DIM, bool DO_K, bool DO_F>
 __global__ __launch_bounds__(CUT_THREADS) void k_cut_elem(AsmArgs a) {
     constexpr int MM = MAX_NT * DIM;
-    const int c = blockIdx.x;
+    const int64_t cl = xcd_logical(blockIdx.x, gridDim.x);
+    if (cl >= a.n_cut) return;  // padding block of the XCD mapping
+    const int c = (int)cl;
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
     __shared__ double s_K[MM * MM];
     __shared__ double s_P[MM];
@@ -464,7 +466,8 @@ __global__ __launch_bounds__(CUT_THREADS) void k_cut_elem(AsmArgs a) {
 template <int DIM, bool DO_K, bool DO_F, typename KT>
 __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     HIP_DYNAMIC_SHARED(double, s_seg)
-    const int blk = blockIdx.x;
+    const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
+    if (blk >= a.n_gblk) return;  // padding block of the XCD mapping
     const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
     const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
     const bool use_lds = DO_K && s1 > s0;
@@ -602,11 +605,12 @@ template <int DIM, bool DO_K, bool DO_F, typename KT>
 static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     if (c.timing_events) (void)hipEventRecord(c.timing_events[0], c.stream);
     if (c.n_cut > 0)
-        hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)c.n_cut), dim3(pgfk::CUT_THREADS), 0, c.stream, a);
+        hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(c.n_cut)),
+                           dim3(pgfk::CUT_THREADS), 0, c.stream, a);
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
-    hipLaunchKernelGGL((pgfk::k_gather<DIM, DO_K, DO_F, KT>), dim3((unsigned)c.n_gblk), dim3(pgfk::GATHER_THREADS), lds,
-                       c.stream, a);
+    hipLaunchKernelGGL((pgfk::k_gather<DIM, DO_K, DO_F, KT>), dim3((unsigned)pgfk::xcd_blocks(c.n_gblk)),
+                       dim3(pgfk::GATHER_THREADS), lds, c.stream, a);
     if (c.timing_events) (void)hipEventRecord(c.timing_events[2], c.stream);
 }
 

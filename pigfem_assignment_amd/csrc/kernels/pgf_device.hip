@@ -63,7 +63,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_off = inc_off;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
-    a.blk_node = blk_node; a.blk_seg = blk_seg;
+    a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.n_cut = n_cut;
     return a;
 }
 

@@ -28,6 +28,16 @@ constexpr int CUT_THREADS = 64;
 // row-gather blocks: threads (one per matrix row) and LDS segment (doubles)
 constexpr int GATHER_THREADS = 128;
 constexpr int SEG_MAX = 4096;
+// MI355X dispatches consecutive workgroups to its 8 XCDs in turn, each XCD
+// with its own L2.  Launching 8*per blocks and mapping block b to logical
+// block (b % 8) * per + b / 8 gives every XCD one contiguous run of logical
+// blocks, so neighbouring rows' element data stay in one L2.
+constexpr int N_XCD = 8;
+__host__ __device__ inline int64_t xcd_blocks(int64_t n) { return (n + N_XCD - 1) / N_XCD * N_XCD; }
+__device__ inline int64_t xcd_logical(int64_t b, int64_t grid) {
+    const int64_t per = grid / N_XCD;
+    return (b % N_XCD) * per + b / N_XCD;
+}
 
 struct CohParams {
     int type;
@@ -36,7 +46,7 @@ struct CohParams {
 
 // Arguments shared by the assembly kernels (passed by value).
 struct AsmArgs {
-    int64_t n_all, n_free, n_rows, n_uncut;
+    int64_t n_all, n_free, n_rows, n_uncut, n_cut;
     const double* X;        // [3][n_all]
     const int32_t* dof;     // [dim][n_all]
     const double* U;        // [dim][n_all]
@@ -82,6 +92,7 @@ struct AsmArgs {
     const int64_t* ke_ptr;
     double* pe;                // cut-element P_e scratch
     const int64_t* pe_ptr;
+    int64_t n_gblk;            // gather blocks
     const int64_t* blk_node;   // [n_gblk+1] first node of each gather block
     const int64_t* blk_seg;    // [n_gblk][2] CSR segment of the block's free rows (empty: not staged)
     // outputs
