@@ -455,50 +455,67 @@ __global__ __launch_bounds__(CUT_THREADS) void k_cut_elem(AsmArgs a) {
     }
 }
 
-// Gather scheme, pass 2: one thread per matrix row.  A block owns a run of
-// consecutive nodes (thread t: node n0 + t/DIM, component t%DIM).  When the
-// free rows of those nodes form one contiguous CSR segment [s0, s1) the
-// segment is accumulated in LDS (each thread owns its row, so no two threads
-// touch one entry) and written back with coalesced stores; otherwise
-// (s0 == s1) the rows are accumulated in place in HBM.  Prescribed rows, only
-// assembled with all_rows, are always accumulated in place.  Each row sums its
-// incident elements in element order, the reference's summation order.
+// Gather scheme, pass 2.  A block owns a run of consecutive nodes; every
+// matrix row of those nodes is served by SL threads (slots), slot s taking the
+// row's incident elements s, s + SL, ... so the element work of one row runs in
+// parallel.  When the free rows of the block form one contiguous CSR segment
+// [s0, s1) it is accumulated in LDS and written back with coalesced stores;
+// otherwise (s0 == s1), and for prescribed rows (all_rows only), the rows are
+// accumulated in place in HBM.  The slots add their element's contribution in
+// slot order between barriers, so every entry sums its incident elements in
+// element order, the reference's summation order, and the result does not
+// depend on scheduling.
+template <int DIM>
+__host__ __device__ constexpr int gather_slots() { return DIM == 2 ? 4 : 8; }
+
 template <int DIM, bool DO_K, bool DO_F, typename KT>
 __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     HIP_DYNAMIC_SHARED(double, s_seg)
+    constexpr int SL = gather_slots<DIM>();
+    __shared__ double s_P[GATHER_THREADS / SL];
     const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
     if (blk >= a.n_gblk) return;  // padding block of the XCD mapping
     const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
     const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
+    const int rounds = a.blk_rounds[blk];
     const bool use_lds = DO_K && s1 > s0;
     const int t = threadIdx.x;
-    if (use_lds)
-        for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) s_seg[j] = 0.0;
-    const int64_t n = n0 + t / DIM;
-    const int da = t % DIM;
-    const bool active = n < n1;
+    const int lr = t / SL, slot = t % SL;  // local row, slot
+    const int64_t n = n0 + lr / DIM;
+    const int da = lr % DIM;
+    const bool active = lr < GATHER_THREADS / SL && n < n1;
     constexpr int S = DIM == 2 ? 3 : 6;
     constexpr int NQ = DIM == 2 ? 4 : 1;
     const int r = active ? a.dof[da * a.n_all + n] : 0;
     const bool own = active && (r < a.n_free || a.all_rows);
     const bool in_lds = use_lds && r < a.n_free;
-    if (DO_K && own && !in_lds)
+    if (use_lds)
+        for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) s_seg[j] = 0.0;
+    if (DO_K && own && !in_lds && slot == 0)
         for (int32_t j = a.rowptr[r]; j < a.rowptr[r + 1]; ++j) {
             if (sizeof(KT) == 8) a.K[j] = 0.0;
             else a.K32[j] = 0.0f;
         }
+    if (DO_F && slot == 0 && lr < GATHER_THREADS / SL) s_P[lr] = 0.0;
     __syncthreads();
-    if (own) {
-        const int64_t base = a.rowptr[r];
-        auto add = [&](int64_t pos, double v) {
-            if (in_lds) s_seg[pos - s0] += v;
-            else kadd<KT>(a, pos, v);
-        };
-        double Pa = 0.0;
-        for (int64_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
+    const int64_t kbeg = own ? a.inc_ptr[n] : 0, kend = own ? a.inc_ptr[n + 1] : 0;
+    const int64_t base = own ? a.rowptr[r] : 0;
+    auto add = [&](int64_t pos, double v) {
+        if (in_lds) s_seg[pos - s0] += v;
+        else kadd<KT>(a, pos, v);
+    };
+    for (int rr = 0; rr < rounds; ++rr) {
+        const int64_t k = kbeg + (int64_t)rr * SL + slot;
+        const bool valid = own && k < kend;
+        // ---- compute: this slot's element contribution to the row ----
+        double Kr[4 * DIM];
+        int prel[4 * DIM];
+        double Pc = 0.0;
+        int c = -1, la = 0;
+        if (valid) {
             const int e = a.inc_elem[k];
-            const int la = a.inc_slot[k];
-            const int c = a.cut_index[e];
+            la = a.inc_slot[k];
+            c = a.cut_index[e];
             if (c < 0) {
                 int nd[4];
                 double X[4][DIM];
@@ -513,7 +530,6 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                 if (DO_F)
                     for (int q = 0; q < 4; ++q)
                         for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
-                double Kr[4 * DIM];
                 for (int j = 0; j < 4 * DIM; ++j) Kr[j] = 0.0;
                 // one quadrature point at a time keeps the register footprint small
 #pragma unroll 1
@@ -529,7 +545,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                         double eps[S], sig[S];
                         strain_of<DIM>(4, g, u, eps);
                         stress_of<DIM>(D, eps, sig);
-                        Pa += w * force_entry<DIM>(gr, da, sig);
+                        Pc += w * force_entry<DIM>(gr, da, sig);
                     }
                     if (DO_K) {
                         double parts[S];
@@ -545,32 +561,42 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                         int cf = 0, cp = 0;
                         for (int db = 0; db < DIM; ++db) {
                             const int cdof = a.dof[db * a.n_all + nd[bn]];
-                            const int64_t pos = cdof < a.n_free ? base + of + cf++ : base + op + cp++;
-                            add(pos, Kr[bn * DIM + db]);
+                            prel[bn * DIM + db] = cdof < a.n_free ? of + cf++ : op + cp++;
                         }
                     }
                 }
-            } else {
-                const int nt = a.ce_nt[c], M = nt * DIM;
-                if (DO_F) Pa += a.pe[a.pe_ptr[c] + la * DIM + da];
+            } else if (DO_F) {
+                Pc = a.pe[a.pe_ptr[c] + la * DIM + da];
+            }
+        }
+        // ---- add, slot by slot (element order) ----
+        for (int ss = 0; ss < SL; ++ss) {
+            if (valid && slot == ss) {
+                if (DO_F) s_P[lr] += Pc;
                 if (DO_K) {
-                    const double* krow = a.ke + a.ke_ptr[c] + (int64_t)(la * DIM + da) * M;
-                    const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
-                    for (int bn = 0; bn < nt; ++bn) {
-                        const int nb = a.ce_nodes[c * 16 + bn];
-                        const int of = off[(la * nt + bn) * 2], op = off[(la * nt + bn) * 2 + 1];
-                        int cf = 0, cp = 0;
-                        for (int db = 0; db < DIM; ++db) {
-                            const int cdof = a.dof[db * a.n_all + nb];
-                            const int64_t pos = cdof < a.n_free ? base + of + cf++ : base + op + cp++;
-                            add(pos, krow[bn * DIM + db]);
+                    if (c < 0) {
+                        for (int j = 0; j < 4 * DIM; ++j) add(base + prel[j], Kr[j]);
+                    } else {
+                        const int nt = a.ce_nt[c], M = nt * DIM;
+                        const double* krow = a.ke + a.ke_ptr[c] + (int64_t)(la * DIM + da) * M;
+                        const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
+                        for (int bn = 0; bn < nt; ++bn) {
+                            const int nb = a.ce_nodes[c * 16 + bn];
+                            const int of = off[(la * nt + bn) * 2], op = off[(la * nt + bn) * 2 + 1];
+                            int cf = 0, cp = 0;
+                            for (int db = 0; db < DIM; ++db) {
+                                const int cdof = a.dof[db * a.n_all + nb];
+                                const int64_t pos = cdof < a.n_free ? base + of + cf++ : base + op + cp++;
+                                add(pos, krow[bn * DIM + db]);
+                            }
                         }
                     }
                 }
             }
+            __syncthreads();
         }
-        if (DO_F) a.P[r] = Pa;
     }
+    if (DO_F && own && slot == 0) a.P[r] = s_P[lr];
     if (use_lds) {
         __syncthreads();
         for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) {

@@ -63,7 +63,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_off = inc_off;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
-    a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.n_cut = n_cut;
+    a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
     return a;
 }
 
@@ -73,7 +73,7 @@ Ctx::~Ctx() {
                     dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, r32, x32, z32, p032, p132, q32, dinv32, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
-                    inc_off, ke, pe, blk_node, blk_seg};
+                    inc_off, ke, pe, blk_node, blk_seg, blk_rounds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -615,11 +615,13 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             // gather blocks: runs of consecutive nodes, one thread per row;
             // a block's free rows are staged in LDS when they form one
             // contiguous CSR segment of at most SEG_MAX entries
-            const int per_blk = pgfk::GATHER_THREADS / d->dim;
+            const int slots = d->dim == 2 ? 4 : 8;  // threads per matrix row (gather_slots)
+            const int per_blk = pgfk::GATHER_THREADS / (slots * d->dim);
             std::vector<int64_t> bn{0}, bs;
+            std::vector<int32_t> brounds;
             int64_t seg_max = 0;
             for (int64_t n0 = 0; n0 < d->n_all_nodes;) {
-                int64_t n1 = n0, rmin = INT64_MAX, rmax = -1, cnt = 0;
+                int64_t n1 = n0, rmin = INT64_MAX, rmax = -1, cnt = 0, inc_max = 0;
                 while (n1 < d->n_all_nodes && n1 - n0 < per_blk) {
                     int64_t mn = rmin, mx = rmax, ct = cnt;
                     for (int dd = 0; dd < d->dim; ++dd) {
@@ -628,8 +630,10 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
                     }
                     if (n1 > n0 && ct > 0 && pat.rowptr[mx + 1] - pat.rowptr[mn] > pgfk::SEG_MAX) break;
                     rmin = mn; rmax = mx; cnt = ct;
+                    inc_max = std::max<int64_t>(inc_max, pat.inc_ptr[n1 + 1] - pat.inc_ptr[n1]);
                     ++n1;
                 }
+                brounds.push_back((int32_t)((inc_max + slots - 1) / slots));
                 int64_t s0 = 0, s1 = 0;
                 if (cnt > 0 && rmax - rmin + 1 == cnt && pat.rowptr[rmax + 1] - pat.rowptr[rmin] <= pgfk::SEG_MAX) {
                     s0 = pat.rowptr[rmin];
@@ -646,6 +650,7 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             if (c->n_gblk > 0x7fffffff) throw pgf::Error(PGF_ELIMIT, "too many gather blocks");
             c->blk_node = pgfd::dupload(bn.data(), (int64_t)bn.size());
             c->blk_seg = pgfd::dupload(bs.data(), std::max<int64_t>((int64_t)bs.size(), 2));
+            c->blk_rounds = pgfd::dupload(brounds.data(), std::max<int64_t>((int64_t)brounds.size(), 1));
         }
         c->dmax_conv = pgfd::dalloc<double>(n_cpt);
         c->dmax_trial = pgfd::dalloc<double>(n_cpt);

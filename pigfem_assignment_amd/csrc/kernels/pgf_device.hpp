@@ -49,6 +49,7 @@ struct Ctx {
     double *ke = nullptr, *pe = nullptr;
     int64_t n_inc = 0, n_ke = 0;
     int64_t *blk_node = nullptr, *blk_seg = nullptr;
+    int32_t* blk_rounds = nullptr;
     int64_t n_gblk = 0, seg_max = 0;
     // free / prescribed dof -> nodal field slot (d*n_all + node)
     int64_t *free_slot = nullptr, *pres_slot = nullptr;

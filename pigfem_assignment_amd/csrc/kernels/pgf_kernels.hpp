@@ -25,8 +25,8 @@ constexpr int max_vpt(int dim) { return dim == 2 ? 32 : 16; }
 constexpr int MAX_CPT = 8;
 // threads of the cut-element kernels (one workgroup per cut element)
 constexpr int CUT_THREADS = 64;
-// row-gather blocks: threads (one per matrix row) and LDS segment (doubles)
-constexpr int GATHER_THREADS = 128;
+// row-gather blocks: threads (slots x matrix rows) and LDS segment (doubles)
+constexpr int GATHER_THREADS = 256;
 constexpr int SEG_MAX = 4096;
 // MI355X dispatches consecutive workgroups to its 8 XCDs in turn, each XCD
 // with its own L2.  Launching 8*per blocks and mapping block b to logical
@@ -95,6 +95,7 @@ struct AsmArgs {
     int64_t n_gblk;            // gather blocks
     const int64_t* blk_node;   // [n_gblk+1] first node of each gather block
     const int64_t* blk_seg;    // [n_gblk][2] CSR segment of the block's free rows (empty: not staged)
+    const int32_t* blk_rounds; // [n_gblk] incidence rounds: max incident elements of a block node / slots
     // outputs
     const int32_t* rowptr;
     double* K;
