@@ -257,7 +257,8 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
 }
 
 // ---------------------------------------------------------------------------
-// Cut element computed by one wavefront into LDS: K_e (M x M) and P_e (M).
+// Cut element computed by one wavefront into LDS: K_e (M x M, row stride M + 1)
+// and P_e (M).
 // Every integration point is staged in LDS first (gradients, J*W, the point's
 // material, its stress; per cohesive point the rotated tangent R^T D_c R, the
 // rotated traction and the surface shape functions), then each lane owns one
@@ -352,7 +353,7 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
         if (tid < M * nsplit) {
             const int i = tid % M, grp = tid / M;
             const int an = i / DIM, da = i % DIM;
-            double* Ki = s_K + i * M;
+            double* Ki = s_K + i * (M + 1);  // row stride M + 1: lanes of different rows hit different banks
             for (int j = grp; j < M; j += nsplit) Ki[j] = 0.0;
             for (int q = 0; q < nv; ++q) {
                 double parts[S];
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(CUT_THREADS) void k_cut(AsmArgs a, int64_t begin, i
     constexpr int MM = MAX_NT * DIM;
     const int c = a.ce_list[li];
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
-    __shared__ double s_K[MM * MM];
+    __shared__ double s_K[MM * (MM + 1)];
     __shared__ double s_P[MM];
     __shared__ int s_node[MAX_NT], s_dof[MM];
     cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(CUT_THREADS) void k_cut(AsmArgs a, int64_t begin, i
             for (int k = 0; k < db; ++k) { if (s_dof[bn * DIM + k] < a.n_free) ++cf; else ++cp; }
             const int cdof = s_dof[j];
             const int64_t pos = cdof < a.n_free ? a.rowptr[r] + of + cf : a.rowptr[r] + op + cp;
-            kadd<KT>(a, pos, s_K[e]);
+            kadd<KT>(a, pos, s_K[i * (M + 1) + j]);
         }
 }
 
@@ -441,13 +442,13 @@ __global__ __launch_bounds__(CUT_THREADS) void k_cut_elem(AsmArgs a) {
     if (cl >= a.n_cut) return;  // padding block of the XCD mapping
     const int c = (int)cl;
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
-    __shared__ double s_K[MM * MM];
+    __shared__ double s_K[MM * (MM + 1)];
     __shared__ double s_P[MM];
     __shared__ int s_node[MAX_NT], s_dof[MM];
     cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
     if (DO_K) {
         double* ke = a.ke + a.ke_ptr[c];
-        for (int e = tid; e < M * M; e += CUT_THREADS) ke[e] = s_K[e];
+        for (int e = tid; e < M * M; e += CUT_THREADS) ke[e] = s_K[(e / M) * (M + 1) + e % M];
     }
     if (DO_F) {
         double* pe = a.pe + a.pe_ptr[c];
