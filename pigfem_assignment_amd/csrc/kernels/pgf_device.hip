@@ -71,7 +71,7 @@ Ctx::~Ctx() {
     void* ptrs[] = {X, dof, U, ue_conn, ue_mat, ue_off, Dmat, ce_list, ce_nodes, ce_nt, vpt_ptr, vpt_grad_off, cpt_ptr,
                     ce_off_ptr, vpt_w, vpt_grad, cpt_w, cpt_N, cpt_R, vpt_mat, cpt_law, cpt_nodes, cpt_nh, ce_off, laws,
                     dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
-                    upres_full, r, x, z, p0, p1, q, dinv, guess, res, r32, x32, z32, p032, p132, q32, dinv32, partial,
+                    upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
                     inc_off, ke, pe, blk_node, blk_seg, blk_rounds};
     for (void* p : ptrs)
@@ -132,10 +132,10 @@ constexpr int RED_BLOCKS = 1024;
 
 // ---------------------------------------------------------------------------
 // SpMV on the free-free block: y = K_ff x, G lanes per row.
-template <typename T, int G>
+template <typename TA, typename T, int G>
 __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
-                                              const T* __restrict__ A, const T* __restrict__ x, T* __restrict__ y) {
+                                              const TA* __restrict__ A, const T* __restrict__ x, T* __restrict__ y) {
     const int lane = threadIdx.x % G;
     for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; row < n;
          row += (int64_t)gridDim.x * blockDim.x / G) {
@@ -148,11 +148,12 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t* __restri
 }
 
 // PCG step A: p_new = z + beta p_old (own rows), q = A p_new (rows gathered on
-// the fly from z and p_old), alpha = rz / (p.q) in the last block.
-template <typename T, int G>
+// the fly from z and p_old), alpha = rz / (p.q) in the last block.  The
+// matrix may be stored in a narrower type than the Krylov vectors.
+template <typename TA, typename T, int G>
 __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int32_t* __restrict__ rowptr,
                                                  const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
-                                                 const T* __restrict__ A, const T* __restrict__ z,
+                                                 const TA* __restrict__ A, const T* __restrict__ z,
                                                  const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
                                                  double* scal, double* partial, unsigned* counter, const int* done) {
     if (*done) return;
@@ -241,9 +242,9 @@ __global__ __launch_bounds__(256) void k_cg_init(int64_t n, const TB* __restrict
     }
 }
 
-template <typename T>
+template <typename TA, typename T>
 __global__ void k_diag_inv(int64_t n, const int32_t* rowptr, const int32_t* rowsplit, const int32_t* col,
-                           const T* A, T* dinv) {
+                           const TA* A, T* dinv) {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         const int32_t b = rowptr[r], e = b + rowsplit[r];
         double d = 0.0;
@@ -278,10 +279,6 @@ __global__ void k_update_U(int64_t n, const int64_t* slot, const T* dx, double* 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         U[slot[i]] += (double)dx[i];
 }
-__global__ void k_to_f32(int64_t n, const double* a, float* b) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = (float)a[i];
-}
 __global__ void k_copy(int64_t n, const double* a, double* b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
 }
@@ -290,11 +287,11 @@ static unsigned grid_for(int64_t n, int per_block = 256) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + per_block - 1) / per_block, RED_BLOCKS));
 }
 
-template <typename T>
-static void enqueue_spmv(Ctx& c, const T* A, const T* x, T* y) {
+template <typename TA, typename T>
+static void enqueue_spmv(Ctx& c, const TA* A, const T* x, T* y) {
     const unsigned nb = (unsigned)std::min<int64_t>((c.n_free * 16 + 255) / 256, 65536);
-    if (c.dim == 3) hipLaunchKernelGGL((k_spmv<T, 16>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
-    else hipLaunchKernelGGL((k_spmv<T, 8>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
+    if (c.dim == 3) hipLaunchKernelGGL((k_spmv<TA, T, 16>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
+    else hipLaunchKernelGGL((k_spmv<TA, T, 8>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
 }
 
 // Enqueue a whole PCG solve in chunks; the device flag stops the recurrence,
@@ -303,16 +300,16 @@ static double wall_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-template <typename T>
-static int run_pcg(Ctx& c, const T* A, const void* b, bool b_is_double, T* x, T* r, T* z, T* p0, T* p1, T* q,
+template <typename TA, typename T>
+static int run_pcg(Ctx& c, const TA* A, const void* b, bool b_is_double, T* x, T* r, T* z, T* p0, T* p1, T* q,
                    T* dinv, bool x0, double rtol, int maxit, pgf_solve_info* info) {
     const int64_t n = c.n_free;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c.stream));
-    hipLaunchKernelGGL((k_diag_inv<T>), dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx, A, dinv);
-    if (x0) enqueue_spmv<T>(c, A, x, q);
+    hipLaunchKernelGGL((k_diag_inv<TA, T>), dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx, A, dinv);
+    if (x0) enqueue_spmv<TA, T>(c, A, x, q);
     else fill_async<T>(c, x, n, (T)0);
     const unsigned gi = grid_for(n);
     if (b_is_double)
@@ -337,10 +334,10 @@ static int run_pcg(Ctx& c, const T* A, const void* b, bool b_is_double, T* x, T*
             T* pold = (it % 2 == 0) ? p0 : p1;
             T* pnew = (it % 2 == 0) ? p1 : p0;
             if (G == 16)
-                hipLaunchKernelGGL((k_cg_spmv<T, 16>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
+                hipLaunchKernelGGL((k_cg_spmv<TA, T, 16>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
                                    A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
             else
-                hipLaunchKernelGGL((k_cg_spmv<T, 8>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
+                hipLaunchKernelGGL((k_cg_spmv<TA, T, 8>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
                                    A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
             hipLaunchKernelGGL((k_cg_update<T>), dim3(gu), dim3(256), 0, c.stream, n, x, r, z, pnew, q, dinv, c.scal,
                                c.partial, c.counter, c.done, (double*)nullptr);
@@ -676,9 +673,6 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         c->r = pgfd::dalloc<double>(nf); c->x = pgfd::dalloc<double>(nf); c->z = pgfd::dalloc<double>(nf);
         c->p0 = pgfd::dalloc<double>(nf); c->p1 = pgfd::dalloc<double>(nf); c->q = pgfd::dalloc<double>(nf);
         c->dinv = pgfd::dalloc<double>(nf); c->guess = pgfd::dalloc<double>(nf); c->res = pgfd::dalloc<double>(nf);
-        c->r32 = pgfd::dalloc<float>(nf); c->x32 = pgfd::dalloc<float>(nf); c->z32 = pgfd::dalloc<float>(nf);
-        c->p032 = pgfd::dalloc<float>(nf); c->p132 = pgfd::dalloc<float>(nf); c->q32 = pgfd::dalloc<float>(nf);
-        c->dinv32 = pgfd::dalloc<float>(nf);
         c->partial = pgfd::dalloc<double>(3 * 2 * pgfd::RED_BLOCKS);
         c->scal = pgfd::dalloc<double>(pgfd::S_NSCAL + 8);
         c->counter = pgfd::dalloc<unsigned>(8);
@@ -849,7 +843,7 @@ PGF_API int pgf_spmv(pgf_ctx* c, const double* x, double* y) {
     if (!c || !x || !y) return PGF_EARG;
     return guard(c, [&]() -> int {
         HIPCHK(hipMemcpyAsync(c->x, x, sizeof(double) * c->n_free, hipMemcpyHostToDevice, c->stream));
-        pgfd::enqueue_spmv<double>(*c, c->K, c->x, c->q);
+        pgfd::enqueue_spmv<double, double>(*c, c->K, c->x, c->q);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, c->q, sizeof(double) * c->n_free, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -863,22 +857,16 @@ PGF_API int pgf_pcg(pgf_ctx* c, const double* b, double* x, const double* x0, do
     return guard(c, [&]() -> int {
         const int64_t n = c->n_free;
         HIPCHK(hipMemcpyAsync(c->res, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-        if (precision == 32) {
-            if (x0) {
-                HIPCHK(hipMemcpyAsync(c->guess, x0, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-                hipLaunchKernelGGL(pgfd::k_to_f32, dim3(pgfd::grid_for(n)), dim3(256), 0, c->stream, n, c->guess, c->x32);
-            }
-            pgfd::run_pcg<float>(*c, c->K32, c->res, true, c->x32, c->r32, c->z32, c->p032, c->p132, c->q32, c->dinv32,
-                                 x0 != nullptr, rtol, maxit, info);
-            std::vector<float> tmp(n);
-            HIPCHK(hipMemcpy(tmp.data(), c->x32, sizeof(float) * n, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; ++i) x[i] = tmp[i];
-        } else {
-            if (x0) HIPCHK(hipMemcpyAsync(c->x, x0, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-            pgfd::run_pcg<double>(*c, c->K, c->res, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, x0 != nullptr,
-                                  rtol, maxit, info);
-            HIPCHK(hipMemcpy(x, c->x, sizeof(double) * n, hipMemcpyDeviceToHost));
-        }
+        if (x0) HIPCHK(hipMemcpyAsync(c->x, x0, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        // precision 32: FP32 matrix values (half the value bytes per SpMV),
+        // FP64 Krylov vectors and recurrences
+        if (precision == 32)
+            pgfd::run_pcg<float, double>(*c, c->K32, c->res, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv,
+                                         x0 != nullptr, rtol, maxit, info);
+        else
+            pgfd::run_pcg<double, double>(*c, c->K, c->res, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv,
+                                          x0 != nullptr, rtol, maxit, info);
+        HIPCHK(hipMemcpy(x, c->x, sizeof(double) * n, hipMemcpyDeviceToHost));
         return 0;
     });
 }
@@ -945,19 +933,17 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
             bool g = o->use_guess == 1 && c->have_guess && it == 0;
             if (c->learn && (it == 0 || o->use_guess == 2 || o->use_guess == 3))
                 g = pgfd::learn_guess(*c, o->use_guess, fp32, it) || g;
-            if (fp32) {
-                if (g) hipLaunchKernelGGL(pgfd::k_to_f32, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x32);
-                pgfd::run_pcg<float>(*c, c->K32, c->r, true, c->x32, c->r32, c->z32, c->p032, c->p132, c->q32, c->dinv32, g,
-                                     lin_tol, o->lin_maxit, &si);
-                hipLaunchKernelGGL((pgfd::k_update_U<float>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
-                                   c->free_slot, c->x32, c->U);
-            } else {
-                if (g) hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x);
-                pgfd::run_pcg<double>(*c, c->K, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g, lin_tol,
-                                      o->lin_maxit, &si);
-                hipLaunchKernelGGL((pgfd::k_update_U<double>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
-                                   c->free_slot, c->x, c->U);
-            }
+            if (g) hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x);
+            // FP32 mode: FP32 tangent values with FP64 Krylov vectors; the
+            // residual, the update and delta_max stay FP64 either way
+            if (fp32)
+                pgfd::run_pcg<float, double>(*c, c->K32, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g,
+                                             lin_tol, o->lin_maxit, &si);
+            else
+                pgfd::run_pcg<double, double>(*c, c->K, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g,
+                                              lin_tol, o->lin_maxit, &si);
+            hipLaunchKernelGGL((pgfd::k_update_U<double>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
+                               c->free_slot, c->x, c->U);
             ms_sol += si.ms;
             pcg_its += si.iterations;
             ++it;
@@ -1048,13 +1034,15 @@ PGF_API int pgf_time_spmv(pgf_ctx* c, int32_t precision, int32_t reps, double* m
     return guard(c, [&]() -> int {
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
-        if (precision == 32) { pgfd::fill_async<float>(*c, c->x32, c->n_free, 1.0f); pgfd::enqueue_spmv<float>(*c, c->K32, c->x32, c->q32); }
-        else { pgfd::fill_async<double>(*c, c->x, c->n_free, 1.0); pgfd::enqueue_spmv<double>(*c, c->K, c->x, c->q); }
+        // precision 32 times the solver's mixed product: FP32 values, FP64 vectors
+        pgfd::fill_async<double>(*c, c->x, c->n_free, 1.0);
+        if (precision == 32) pgfd::enqueue_spmv<float, double>(*c, c->K32, c->x, c->q);
+        else pgfd::enqueue_spmv<double, double>(*c, c->K, c->x, c->q);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipEventRecord(e0, c->stream));
         for (int i = 0; i < reps; ++i) {
-            if (precision == 32) pgfd::enqueue_spmv<float>(*c, c->K32, c->x32, c->q32);
-            else pgfd::enqueue_spmv<double>(*c, c->K, c->x, c->q);
+            if (precision == 32) pgfd::enqueue_spmv<float, double>(*c, c->K32, c->x, c->q);
+            else pgfd::enqueue_spmv<double, double>(*c, c->K, c->x, c->q);
         }
         HIPCHK(hipEventRecord(e1, c->stream));
         HIPCHK(hipEventSynchronize(e1));

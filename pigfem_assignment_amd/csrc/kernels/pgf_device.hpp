@@ -63,8 +63,6 @@ struct Ctx {
     // solver work (FP64 and FP32)
     double *r = nullptr, *x = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr,
            *guess = nullptr, *res = nullptr;
-    float *r32 = nullptr, *x32 = nullptr, *z32 = nullptr, *p032 = nullptr, *p132 = nullptr, *q32 = nullptr,
-          *dinv32 = nullptr;
     double* partial = nullptr;     // block partial sums
     double* scal = nullptr;        // device scalars of the PCG recurrence
     unsigned* counter = nullptr;   // last-block counters
