@@ -41,6 +41,21 @@ def gather_bytes(dim, nnz_rows, n_rows, n_all, n_elem):
     return 8 * nnz_rows + 8 * n_rows + 2 * 8 * dim * n_all + 17 * n_elem
 
 
+def tune_gather_slots(ctx, flags, reps=5, log=print):
+    """Time the gather pass at 1, 2, 4 and 8 threads per matrix row (results
+    are bit-identical) and keep the fastest; untimed set-up, like an autotuner."""
+    ms = {}
+    for slots in (1, 2, 4, 8):
+        ctx.gather_slots = slots
+        ctx.assemble(flags)
+        ms[slots] = ctx.time_assemble_split(flags, reps=reps)[2]
+    best = min(ms, key=ms.get)
+    ctx.gather_slots = best
+    log("# gather pass ms by threads per row: %s -> %d" % (
+        ", ".join("%d: %.3f" % (k, v) for k, v in ms.items()), best))
+    return best, ms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +102,7 @@ def main():
     print("# device context ready: nnz %d, %d colours" % (ctx.csr.nnz, ctx.csr.n_colors), flush=True)
 
     flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
+    slots, slot_ms = tune_gather_slots(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags)
     ms = ctx.time_assemble(flags, reps=args.steps)
@@ -105,7 +121,8 @@ def main():
     print("# assembly K+f FP64 (coloured RMW): %.3f ms/step, %.1f GB/s algorithmic (RMW bytes)" % (
         ms_col, info.n_elem * algorithmic_bytes_per_elem(2) / (ms_col * 1e-3) / 1e9), flush=True)
 
-    extra = {"assembly_gather_ms": ms, "assembly_gather_cut_pass_ms": ms_cut,
+    extra = {"gather_threads_per_row": slots, "gather_pass_ms_by_threads_per_row": slot_ms,
+             "assembly_gather_ms": ms, "assembly_gather_cut_pass_ms": ms_cut,
              "assembly_gather_pass_ms": ms_gather, "assembly_gather_bytes": bytes_gather,
              "assembly_colored_ms": ms_col,
              "assembly_colored_elem_per_s": info.n_elem / (ms_col * 1e-3),
@@ -269,6 +286,7 @@ def bench_c2(args):
         info.n_elem, info.n_cut, info.n_free, info.n_cpt, time.time() - t0), flush=True)
     ctx = DeviceContext(prob)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
+    slots, slot_ms = tune_gather_slots(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags, dt=0.05)
     ms = ctx.time_assemble(flags, reps=args.steps)
@@ -278,9 +296,11 @@ def bench_c2(args):
     achieved = bytes_gather / (ms_gather * 1e-3) / 1e9
     print("# C2 assembly K+f: %.2f ms (cut pass %.2f, gather %.2f), %.1f GB/s" % (ms, ms_cut, ms_gather, achieved),
           flush=True)
-    extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather}
-    extra.update(learned_vs_plain(args.c2_n, args.c2_incl, args.c2_steps, prob=(prob, incl),
-                                  log=lambda m: print(m, flush=True), ctx=ctx, modes=args.c2_modes.split(",")))
+    extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather,
+             "gather_threads_per_row": slots, "gather_pass_ms_by_threads_per_row": slot_ms}
+    if not args.no_learn:
+        extra.update(learned_vs_plain(args.c2_n, args.c2_incl, args.c2_steps, prob=(prob, incl),
+                                      log=lambda m: print(m, flush=True), ctx=ctx, modes=args.c2_modes.split(",")))
     line = {
         "metric": "element assemblies/s; time-to-1e-8 residual per load step, learned vs plain",
         "value": info.n_elem / (ms * 1e-3), "unit": "element assemblies/s", "n_gpus": 1, "steps": args.steps,
@@ -309,6 +329,7 @@ def bench_c3(args):
         len(parts), B.n_elem, B.n_cut, B.n_free, time.time() - t0), flush=True)
     ctx = DeviceContext(B)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS
+    slots, _ = tune_gather_slots(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags)
     ms_asm = ctx.time_assemble(flags, reps=args.steps)
@@ -331,7 +352,7 @@ def bench_c3(args):
                    "elements": B.n_elem, "dofs": B.n_free},
         "roofline": None, "cpu_baseline": None,
         "extra": {"batched_solve_ms": ms_solve, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
-                  "iterations_total": int(its.sum()), "autotuned": list(tune),
+                  "iterations_total": int(its.sum()), "autotuned": list(tune), "gather_threads_per_row": slots,
                   "C_eff_mean": Ce.mean(axis=0).tolist()},
     }
     print(json.dumps(line))

@@ -223,8 +223,15 @@ PGF_API int pgf_newton_step(pgf_ctx* ctx, double frac, const pgf_newton_opts* op
 PGF_API int pgf_time_assemble(pgf_ctx* ctx, uint32_t flags, int32_t reps, double* ms_per_call);
 /* per-pass device time of the row-gather assembly (out[0] call, out[1] cut-element pass, out[2] gather pass) */
 PGF_API int pgf_time_assemble_split(pgf_ctx* ctx, uint32_t flags, int32_t reps, double* out);
-/* time `reps` K_ff SpMVs (FP64 or FP32) on device-resident vectors */
+/* time `reps` K_ff SpMVs on device-resident FP64 vectors, FP64 or FP32 values */
 PGF_API int pgf_time_spmv(pgf_ctx* ctx, int32_t precision, int32_t reps, double* ms_per_call);
+
+/* ---- tuning of the row-gather assembly (no counterpart in the reference:
+ * the reference assembles with MatSetValues, src/Assembler.cpp:47-184) ----
+ * threads per matrix row of the gather pass: 1, 2, 4 or 8 (default 1 in 2D,
+ * 4 in 3D); rebuilds the node-block partition, results do not change */
+PGF_API int pgf_set_gather_slots(pgf_ctx* ctx, int32_t slots);
+PGF_API int pgf_get_gather_slots(pgf_ctx* ctx, int32_t* slots);
 
 #ifdef __cplusplus
 }

@@ -272,6 +272,17 @@ class DeviceContext:
         self._chk(self.lib.pgf_time_assemble_split(self.h, flags, reps, L.ptr(out, C.c_double)))
         return tuple(out)
 
+    @property
+    def gather_slots(self):
+        """Threads per matrix row of the row-gather assembly pass."""
+        v = C.c_int32()
+        self._chk(self.lib.pgf_get_gather_slots(self.h, C.byref(v)))
+        return v.value
+
+    @gather_slots.setter
+    def gather_slots(self, slots):
+        self._chk(self.lib.pgf_set_gather_slots(self.h, int(slots)))
+
     def time_spmv(self, precision=64, reps=20):
         ms = C.c_double()
         self._chk(self.lib.pgf_time_spmv(self.h, precision, reps, C.byref(ms)))

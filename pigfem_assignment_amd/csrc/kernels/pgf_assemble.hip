@@ -457,22 +457,19 @@ __global__ __launch_bounds__(CUT_THREADS) void k_cut_elem(AsmArgs a) {
 }
 
 // Gather scheme, pass 2.  A block owns a run of consecutive nodes; every
-// matrix row of those nodes is served by SL threads (slots), slot s taking the
-// row's incident elements s, s + SL, ... so the element work of one row runs in
-// parallel.  When the free rows of the block form one contiguous CSR segment
+// matrix row of those nodes is served by SL threads (slots, 1, 2, 4 or 8, a
+// per-context choice), slot s taking the row's incident elements s, s + SL, ...
+// so the element work of one row runs in parallel.  When the free rows of the block form one contiguous CSR segment
 // [s0, s1) it is accumulated in LDS and written back with coalesced stores;
 // otherwise (s0 == s1), and for prescribed rows (all_rows only), the rows are
 // accumulated in place in HBM.  The slots add their element's contribution in
 // slot order between barriers, so every entry sums its incident elements in
 // element order, the reference's summation order, and the result does not
-// depend on scheduling.
-template <int DIM>
-__host__ __device__ constexpr int gather_slots() { return DIM == 2 ? 4 : 8; }
-
-template <int DIM, bool DO_K, bool DO_F, typename KT>
+// depend on scheduling (with one slot a thread owns its row and needs no
+// barriers between elements).
+template <int DIM, int SL, bool DO_K, bool DO_F, typename KT>
 __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     HIP_DYNAMIC_SHARED(double, s_seg)
-    constexpr int SL = gather_slots<DIM>();
     __shared__ double s_P[GATHER_THREADS / SL];
     const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
     if (blk >= a.n_gblk) return;  // padding block of the XCD mapping
@@ -594,7 +591,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                     }
                 }
             }
-            __syncthreads();
+            if (SL > 1) __syncthreads();
         }
     }
     if (DO_F && own && slot == 0) a.P[r] = s_P[lr];
@@ -636,8 +633,13 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
                            dim3(pgfk::CUT_THREADS), 0, c.stream, a);
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
-    hipLaunchKernelGGL((pgfk::k_gather<DIM, DO_K, DO_F, KT>), dim3((unsigned)pgfk::xcd_blocks(c.n_gblk)),
-                       dim3(pgfk::GATHER_THREADS), lds, c.stream, a);
+    const dim3 grid((unsigned)pgfk::xcd_blocks(c.n_gblk)), blk(pgfk::GATHER_THREADS);
+    switch (c.gslots) {
+        case 1: hipLaunchKernelGGL((pgfk::k_gather<DIM, 1, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
+        case 2: hipLaunchKernelGGL((pgfk::k_gather<DIM, 2, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
+        case 4: hipLaunchKernelGGL((pgfk::k_gather<DIM, 4, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
+        default: hipLaunchKernelGGL((pgfk::k_gather<DIM, 8, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
+    }
     if (c.timing_events) (void)hipEventRecord(c.timing_events[2], c.stream);
 }
 

@@ -125,6 +125,60 @@ __device__ inline bool last_block(double* v, double* partial, unsigned* counter)
     return true;
 }
 
+// Gather blocks: runs of consecutive nodes, `slots` threads per matrix row,
+// GATHER_THREADS / (slots * dim) nodes per block.  A block's free rows are
+// staged in LDS when they form one contiguous CSR segment of at most SEG_MAX
+// entries; blk_rounds holds the incident elements of the block's busiest node
+// divided by the slots (rounded up).
+int default_gather_slots(int dim) { return dim == 2 ? 1 : 4; }
+
+void build_gather_blocks(Ctx& c, int slots) {
+    if (slots != 1 && slots != 2 && slots != 4 && slots != 8)
+        throw pgf::Error(PGF_EARG, "gather slots must be 1, 2, 4 or 8");
+    const int per_blk = pgfk::GATHER_THREADS / (slots * c.dim);
+    const int64_t na = c.n_all;
+    const std::vector<int64_t>& rp = c.h_rowptr;
+    std::vector<int64_t> bn{0}, bs;
+    std::vector<int32_t> brounds;
+    int64_t seg_max = 0;
+    for (int64_t n0 = 0; n0 < na;) {
+        int64_t n1 = n0, rmin = INT64_MAX, rmax = -1, cnt = 0, inc_max = 0;
+        while (n1 < na && n1 - n0 < per_blk) {
+            int64_t mn = rmin, mx = rmax, ct = cnt;
+            for (int dd = 0; dd < c.dim; ++dd) {
+                const int64_t g = c.h_dof[(size_t)dd * na + n1];
+                if (g < c.n_free) { mn = std::min(mn, g); mx = std::max(mx, g); ++ct; }
+            }
+            if (n1 > n0 && ct > 0 && rp[mx + 1] - rp[mn] > pgfk::SEG_MAX) break;
+            rmin = mn; rmax = mx; cnt = ct;
+            inc_max = std::max<int64_t>(inc_max, c.h_inc_ptr[n1 + 1] - c.h_inc_ptr[n1]);
+            ++n1;
+        }
+        brounds.push_back((int32_t)((inc_max + slots - 1) / slots));
+        int64_t s0 = 0, s1 = 0;
+        if (cnt > 0 && rmax - rmin + 1 == cnt && rp[rmax + 1] - rp[rmin] <= pgfk::SEG_MAX) {
+            s0 = rp[rmin];
+            s1 = rp[rmax + 1];
+            seg_max = std::max(seg_max, s1 - s0);
+        }
+        bs.push_back(s0);
+        bs.push_back(s1);
+        bn.push_back(n1);
+        n0 = n1;
+    }
+    if ((int64_t)bn.size() - 1 > 0x7fffffff) throw pgf::Error(PGF_ELIMIT, "too many gather blocks");
+    int64_t* node = dupload(bn.data(), (int64_t)bn.size());
+    int64_t* seg = dupload(bs.data(), std::max<int64_t>((int64_t)bs.size(), 2));
+    int32_t* rounds = dupload(brounds.data(), std::max<int64_t>((int64_t)brounds.size(), 1));
+    if (c.blk_node) HIPCHK(hipFree(c.blk_node));
+    if (c.blk_seg) HIPCHK(hipFree(c.blk_seg));
+    if (c.blk_rounds) HIPCHK(hipFree(c.blk_rounds));
+    c.blk_node = node; c.blk_seg = seg; c.blk_rounds = rounds;
+    c.n_gblk = (int64_t)bn.size() - 1;
+    c.seg_max = seg_max;
+    c.gslots = slots;
+}
+
 // scalar slots of the PCG recurrence
 enum { S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_RR0, S_TOL2, S_ITER, S_MAXIT, S_BB, S_NSCAL = 16 };
 
@@ -609,45 +663,9 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->pe_ptr = pgfd::dupload(pp.data(), d->n_cut + 1);
             c->ke = pgfd::dalloc<double>(kp[d->n_cut]);
             c->pe = pgfd::dalloc<double>(pp[d->n_cut]);
-            // gather blocks: runs of consecutive nodes, one thread per row;
-            // a block's free rows are staged in LDS when they form one
-            // contiguous CSR segment of at most SEG_MAX entries
-            const int slots = d->dim == 2 ? 4 : 8;  // threads per matrix row (gather_slots)
-            const int per_blk = pgfk::GATHER_THREADS / (slots * d->dim);
-            std::vector<int64_t> bn{0}, bs;
-            std::vector<int32_t> brounds;
-            int64_t seg_max = 0;
-            for (int64_t n0 = 0; n0 < d->n_all_nodes;) {
-                int64_t n1 = n0, rmin = INT64_MAX, rmax = -1, cnt = 0, inc_max = 0;
-                while (n1 < d->n_all_nodes && n1 - n0 < per_blk) {
-                    int64_t mn = rmin, mx = rmax, ct = cnt;
-                    for (int dd = 0; dd < d->dim; ++dd) {
-                        const int64_t g = d->dof[(size_t)dd * d->n_all_nodes + n1];
-                        if (g < d->n_free) { mn = std::min(mn, g); mx = std::max(mx, g); ++ct; }
-                    }
-                    if (n1 > n0 && ct > 0 && pat.rowptr[mx + 1] - pat.rowptr[mn] > pgfk::SEG_MAX) break;
-                    rmin = mn; rmax = mx; cnt = ct;
-                    inc_max = std::max<int64_t>(inc_max, pat.inc_ptr[n1 + 1] - pat.inc_ptr[n1]);
-                    ++n1;
-                }
-                brounds.push_back((int32_t)((inc_max + slots - 1) / slots));
-                int64_t s0 = 0, s1 = 0;
-                if (cnt > 0 && rmax - rmin + 1 == cnt && pat.rowptr[rmax + 1] - pat.rowptr[rmin] <= pgfk::SEG_MAX) {
-                    s0 = pat.rowptr[rmin];
-                    s1 = pat.rowptr[rmax + 1];
-                    seg_max = std::max(seg_max, s1 - s0);
-                }
-                bs.push_back(s0);
-                bs.push_back(s1);
-                bn.push_back(n1);
-                n0 = n1;
-            }
-            c->n_gblk = (int64_t)bn.size() - 1;
-            c->seg_max = seg_max;
-            if (c->n_gblk > 0x7fffffff) throw pgf::Error(PGF_ELIMIT, "too many gather blocks");
-            c->blk_node = pgfd::dupload(bn.data(), (int64_t)bn.size());
-            c->blk_seg = pgfd::dupload(bs.data(), std::max<int64_t>((int64_t)bs.size(), 2));
-            c->blk_rounds = pgfd::dupload(brounds.data(), std::max<int64_t>((int64_t)brounds.size(), 1));
+            c->h_dof.assign(d->dof, d->dof + (size_t)d->dim * d->n_all_nodes);
+            c->h_inc_ptr = pat.inc_ptr;
+            pgfd::build_gather_blocks(*c, pgfd::default_gather_slots(d->dim));
         }
         c->dmax_conv = pgfd::dalloc<double>(n_cpt);
         c->dmax_trial = pgfd::dalloc<double>(n_cpt);
@@ -1027,6 +1045,21 @@ PGF_API int pgf_time_assemble_split(pgf_ctx* c, uint32_t flags, int32_t reps, do
         (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
         return 0;
     });
+}
+
+PGF_API int pgf_set_gather_slots(pgf_ctx* c, int32_t slots) {
+    if (!c) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        pgfd::build_gather_blocks(*c, slots);
+        return 0;
+    });
+}
+
+PGF_API int pgf_get_gather_slots(pgf_ctx* c, int32_t* slots) {
+    if (!c || !slots) return PGF_EARG;
+    *slots = c->gslots;
+    return 0;
 }
 
 PGF_API int pgf_time_spmv(pgf_ctx* c, int32_t precision, int32_t reps, double* ms_per_call) {

@@ -51,6 +51,9 @@ struct Ctx {
     int64_t *blk_node = nullptr, *blk_seg = nullptr;
     int32_t* blk_rounds = nullptr;
     int64_t n_gblk = 0, seg_max = 0;
+    int gslots = 1;                 // threads per matrix row in the gather pass
+    std::vector<int32_t> h_dof;      // [dim][n_all] (gather partition)
+    std::vector<int64_t> h_inc_ptr;  // [n_all+1] node -> element incidence
     // free / prescribed dof -> nodal field slot (d*n_all + node)
     int64_t *free_slot = nullptr, *pres_slot = nullptr;
     // matrix
@@ -79,6 +82,8 @@ struct Ctx {
 template <typename T>
 void fill_async(Ctx& c, T* p, int64_t n, T v);
 void enqueue_assembly(Ctx& c, double dt, uint32_t flags);
+int default_gather_slots(int dim);
+void build_gather_blocks(Ctx& c, int slots);
 
 // Learned initial guess (pgf_cnn.hip).  Called after the Jacobian assembly of
 // Newton iteration `it` with that iteration's residual in c.r.  Iteration 0
