@@ -24,46 +24,57 @@
 
 namespace pgfk {
 
+// An isotropic D (src/material_lei.cpp:52-97, every material of the path)
+// has three distinct entries: a on the normal diagonal, b off the diagonal of
+// the normal block, c on the shear diagonal (plane strain, plane stress and
+// 3D alike).  The products below are the full D products with the zero terms
+// left out; the nonzero terms keep the full products' order, so the results
+// are the same while the registers hold three numbers instead of S*S.
+struct Iso {
+    double a, b, c;
+};
+template <int DIM>
+__device__ inline Iso iso_of(const double* D) {
+    constexpr int S = DIM == 2 ? 3 : 6;
+    return Iso{D[0], D[1], D[S * S - 1]};
+}
+
 // parts[k] = sum_l B_l,(a,da) D_lk for Voigt row k, the reference's part0..5
 template <int DIM>
-__device__ inline void k_parts(const double* g, int da, const double* D, double* parts) {
+__device__ inline void k_parts(const double* g, int da, const Iso& m, double* parts) {
     if (DIM == 2) {
-        if (da == 0)
-            for (int k = 0; k < 3; ++k) parts[k] = g[0] * D[0 * 3 + k] + g[1] * D[2 * 3 + k];
-        else
-            for (int k = 0; k < 3; ++k) parts[k] = g[1] * D[1 * 3 + k] + g[0] * D[2 * 3 + k];
+        if (da == 0) { parts[0] = g[0] * m.a; parts[1] = g[0] * m.b; parts[2] = g[1] * m.c; }
+        else         { parts[0] = g[1] * m.b; parts[1] = g[1] * m.a; parts[2] = g[0] * m.c; }
     } else {
-        if (da == 0)
-            for (int k = 0; k < 6; ++k) parts[k] = g[0] * D[0 * 6 + k] + g[2] * D[4 * 6 + k] + g[1] * D[5 * 6 + k];
-        else if (da == 1)
-            for (int k = 0; k < 6; ++k) parts[k] = g[1] * D[1 * 6 + k] + g[2] * D[3 * 6 + k] + g[0] * D[5 * 6 + k];
-        else
-            for (int k = 0; k < 6; ++k) parts[k] = g[2] * D[2 * 6 + k] + g[1] * D[3 * 6 + k] + g[0] * D[4 * 6 + k];
+        if (da == 0) {
+            parts[0] = g[0] * m.a; parts[1] = g[0] * m.b; parts[2] = g[0] * m.b;
+            parts[3] = 0.0;        parts[4] = g[2] * m.c; parts[5] = g[1] * m.c;
+        } else if (da == 1) {
+            parts[0] = g[1] * m.b; parts[1] = g[1] * m.a; parts[2] = g[1] * m.b;
+            parts[3] = g[2] * m.c; parts[4] = 0.0;        parts[5] = g[0] * m.c;
+        } else {
+            parts[0] = g[2] * m.b; parts[1] = g[2] * m.b; parts[2] = g[2] * m.a;
+            parts[3] = g[1] * m.c; parts[4] = g[0] * m.c; parts[5] = 0.0;
+        }
     }
 }
-// The same parts with the node's Voigt B column built first (zeros included)
-// and multiplied through in row order; the nonzero terms are summed in the
-// order k_parts uses.  Used where da is a run-time value inside a loop.
+// The same parts for a run-time da inside a loop: the normal entry of the
+// node's B column is g[da] (the others zero), the shear entries select
+// without branching on da.
 template <int DIM>
-__device__ inline void k_parts_col(const double* g, int da, const double* D, double* parts) {
-    constexpr int S = DIM == 2 ? 3 : 6;
-    double b[S];
+__device__ inline void k_parts_col(const double* g, int da, const Iso& m, double* parts) {
+    const double gn = DIM == 2 ? (da == 0 ? g[0] : g[1]) : (da == 0 ? g[0] : (da == 1 ? g[1] : g[2]));
     if (DIM == 2) {
-        b[0] = da == 0 ? g[0] : 0.0;
-        b[1] = da == 1 ? g[1] : 0.0;
-        b[2] = da == 0 ? g[1] : g[0];
+        parts[0] = gn * (da == 0 ? m.a : m.b);
+        parts[1] = gn * (da == 1 ? m.a : m.b);
+        parts[2] = (da == 0 ? g[1] : g[0]) * m.c;
     } else {
-        b[0] = da == 0 ? g[0] : 0.0;
-        b[1] = da == 1 ? g[1] : 0.0;
-        b[2] = da == 2 ? g[2] : 0.0;
-        b[3] = da == 0 ? 0.0 : (da == 1 ? g[2] : g[1]);
-        b[4] = da == 1 ? 0.0 : (da == 0 ? g[2] : g[0]);
-        b[5] = da == 2 ? 0.0 : (da == 0 ? g[1] : g[0]);
-    }
-    for (int k = 0; k < S; ++k) {
-        double s = 0.0;
-        for (int l = 0; l < S; ++l) s += b[l] * D[l * S + k];
-        parts[k] = s;
+        parts[0] = gn * (da == 0 ? m.a : m.b);
+        parts[1] = gn * (da == 1 ? m.a : m.b);
+        parts[2] = gn * (da == 2 ? m.a : m.b);
+        parts[3] = da == 0 ? 0.0 : (da == 1 ? g[2] : g[1]) * m.c;
+        parts[4] = da == 1 ? 0.0 : (da == 0 ? g[2] : g[0]) * m.c;
+        parts[5] = da == 2 ? 0.0 : (da == 0 ? g[1] : g[0]) * m.c;
     }
 }
 // K(row, (b,db)) from parts and column node gradient gb
@@ -93,18 +104,18 @@ __device__ inline void strain_of(int nn, const double* g, const double* u, doubl
     }
 }
 template <int DIM>
-__device__ inline void stress_of(const double* D, const double* e, double* s) {
+__device__ inline void stress_of(const Iso& m, const double* e, double* s) {
     if (DIM == 2) {
-        s[0] = D[0] * e[0] + D[1] * e[1];
-        s[1] = D[3] * e[0] + D[4] * e[1];
-        s[2] = D[8] * e[2];
+        s[0] = m.a * e[0] + m.b * e[1];
+        s[1] = m.b * e[0] + m.a * e[1];
+        s[2] = m.c * e[2];
     } else {
-        s[0] = D[0] * e[0] + D[1] * e[1] + D[2] * e[2];
-        s[1] = D[6] * e[0] + D[7] * e[1] + D[8] * e[2];
-        s[2] = D[12] * e[0] + D[13] * e[1] + D[14] * e[2];
-        s[3] = D[21] * e[3];
-        s[4] = D[28] * e[4];
-        s[5] = D[35] * e[5];
+        s[0] = m.a * e[0] + m.b * e[1] + m.b * e[2];
+        s[1] = m.b * e[0] + m.a * e[1] + m.b * e[2];
+        s[2] = m.b * e[0] + m.b * e[1] + m.a * e[2];
+        s[3] = m.c * e[3];
+        s[4] = m.c * e[4];
+        s[5] = m.c * e[5];
     }
 }
 // B^T sigma for one node (SmallStrainInternalForceFast)
@@ -198,8 +209,7 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
         for (int i = 0; i < DIM; ++i) X[k][i] = a.X[i * a.n_all + nd[k]];
     }
     const int mat = a.ue_mat[p];
-    double D[S * S];
-    for (int k = 0; k < S * S; ++k) D[k] = a.Dmat[mat * 36 + k];
+    const Iso D = iso_of<DIM>(a.Dmat + mat * 36);
     double g[NQ][4 * DIM], w[NQ];
     uncut_geometry<DIM>(X, a.tet_scale, g, w);
     int row_dof[4][DIM];
@@ -303,7 +313,7 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
     if (DO_F && tid < nv) {
         double eps[S];
         strain_of<DIM>(nt, &s_g[tid * MM], s_u, eps);
-        stress_of<DIM>(a.Dmat + s_vmat[tid] * 36, eps, &s_sig[tid * S]);
+        stress_of<DIM>(iso_of<DIM>(a.Dmat + s_vmat[tid] * 36), eps, &s_sig[tid * S]);
     }
     if (tid < nc) {
         const int64_t q = c0 + tid;
@@ -357,7 +367,7 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
             for (int j = grp; j < M; j += nsplit) Ki[j] = 0.0;
             for (int q = 0; q < nv; ++q) {
                 double parts[S];
-                k_parts_col<DIM>(&s_g[q * MM + an * DIM], da, a.Dmat + s_vmat[q] * 36, parts);
+                k_parts_col<DIM>(&s_g[q * MM + an * DIM], da, iso_of<DIM>(a.Dmat + s_vmat[q] * 36), parts);
                 const double wq = s_w[q];
                 const double* gq = &s_g[q * MM];
                 for (int j = grp; j < M; j += nsplit) Ki[j] += wq * k_entry<DIM>(parts, gq + (j / DIM) * DIM, j % DIM);
@@ -507,7 +517,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         const bool valid = own && k < kend;
         // ---- compute: this slot's element contribution to the row ----
         double Kr[4 * DIM];
-        int prel[4 * DIM];
+        int nd[4];
         double Pc = 0.0;
         int c = -1, la = 0;
         if (valid) {
@@ -515,15 +525,13 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
             la = a.inc_slot[k];
             c = a.cut_index[e];
             if (c < 0) {
-                int nd[4];
                 double X[4][DIM];
                 for (int q = 0; q < 4; ++q) {
                     nd[q] = a.el_conn[(int64_t)q * a.n_elem + e];
                     for (int i = 0; i < DIM; ++i) X[q][i] = a.X[i * a.n_all + nd[q]];
                 }
                 const int mat = a.el_mat[e];
-                double D[S * S];
-                for (int q = 0; q < S * S; ++q) D[q] = a.Dmat[mat * 36 + q];
+                const Iso D = iso_of<DIM>(a.Dmat + mat * 36);
                 double u[4 * DIM];
                 if (DO_F)
                     for (int q = 0; q < 4; ++q)
@@ -552,17 +560,6 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                             for (int db = 0; db < DIM; ++db) Kr[bn * DIM + db] += w * k_entry<DIM>(parts, &g[bn * DIM], db);
                     }
                 }
-                if (DO_K) {
-                    const uint64_t off = a.inc_off[k];
-                    for (int bn = 0; bn < 4; ++bn) {
-                        const int of = (int)((off >> (16 * bn)) & 0xff), op = (int)((off >> (16 * bn + 8)) & 0xff);
-                        int cf = 0, cp = 0;
-                        for (int db = 0; db < DIM; ++db) {
-                            const int cdof = a.dof[db * a.n_all + nd[bn]];
-                            prel[bn * DIM + db] = cdof < a.n_free ? of + cf++ : op + cp++;
-                        }
-                    }
-                }
             } else if (DO_F) {
                 Pc = a.pe[a.pe_ptr[c] + la * DIM + da];
             }
@@ -573,7 +570,17 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                 if (DO_F) s_P[lr] += Pc;
                 if (DO_K) {
                     if (c < 0) {
-                        for (int j = 0; j < 4 * DIM; ++j) add(base + prel[j], Kr[j]);
+                        // column positions from the packed per-node offsets
+                        // (free and prescribed part of the row)
+                        const uint64_t off = a.inc_off[k];
+                        for (int bn = 0; bn < 4; ++bn) {
+                            const int of = (int)((off >> (16 * bn)) & 0xff), op = (int)((off >> (16 * bn + 8)) & 0xff);
+                            int cf = 0, cp = 0;
+                            for (int db = 0; db < DIM; ++db) {
+                                const int cdof = a.dof[db * a.n_all + nd[bn]];
+                                add(cdof < a.n_free ? base + of + cf++ : base + op + cp++, Kr[bn * DIM + db]);
+                            }
+                        }
                     } else {
                         const int nt = a.ce_nt[c], M = nt * DIM;
                         const double* krow = a.ke + a.ke_ptr[c] + (int64_t)(la * DIM + da) * M;
