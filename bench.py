@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--c2-n", type=int, default=128)
     ap.add_argument("--c2-incl", type=int, default=200)
     ap.add_argument("--c2-steps", type=int, default=2, help="load increments timed (of 20 to 2%% strain)")
-    ap.add_argument("--c2-modes", default="plain_fp64,learned_fp32_forcing",
+    ap.add_argument("--c2-modes", default="plain_fp64,learned_fp64_forcing,learned_fp32_forcing",
                     help="comma list of " + ",".join(ALL_MODES))
     ap.add_argument("--c3-rves", type=int, default=256)
     ap.add_argument("--c3-nmin", type=int, default=64)
@@ -196,12 +196,13 @@ def main():
     print(json.dumps(line))
 
 
-# (use_guess, precision, PCG tolerance floor, forcing term).  The FP32 mode
-# solves with FP32 tangent values (FP64 Krylov vectors) to 1e-3 relative per
-# Newton iteration; the FP64 Newton residual refines, as in iterative
-# refinement, down to the 1e-8 target.
-ALL_MODES = {"plain_fp64": (0, 64, 1e-10, False), "plain_fp64_forcing": (0, 64, 1e-10, True),
-             "learned_fp64_forcing": (2, 64, 1e-10, True), "learned_fp32_forcing": (2, 32, 1e-3, True)}
+# (use_guess, precision, PCG tolerance floor, forcing term, FP32 iteration
+# budget).  The FP32 mode solves with FP32 tangent values (FP64 Krylov
+# vectors) to 1e-3 relative per Newton iteration and the FP64 Newton residual
+# refines, as in iterative refinement, down to the 1e-8 target; a solve that
+# has not converged after 2000 FP32 iterations continues in FP64.
+ALL_MODES = {"plain_fp64": (0, 64, 1e-10, False, 0), "plain_fp64_forcing": (0, 64, 1e-10, True, 0),
+             "learned_fp64_forcing": (2, 64, 1e-10, True, 0), "learned_fp32_forcing": (2, 32, 1e-3, True, 2000)}
 
 
 def copy_probe():
@@ -253,21 +254,24 @@ def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, tra
     # the inexact-Newton forcing term (each solve only as tight as needed)
     modes = [(m,) + ALL_MODES[m] for m in modes]
     flags = {}
-    for name, ug, prec, lin, adapt in modes:
+    for name, ug, prec, lin, adapt, m32 in modes:
         recs, fl, ctx = learn.run_steps(prob[0], prob[1], steps, dfrac, use_guess=ug, net=net, lin_rtol=lin,
-                                        precision=prec, ctx=ctx, adaptive_lin=adapt)
+                                        precision=prec, ctx=ctx, adaptive_lin=adapt, fp32_maxit=m32)
         flags[name] = fl
         out["%s_ms_per_step" % name] = [r["ms"] for r in recs]
         out["%s_pcg_per_step" % name] = [r["pcg"] for r in recs]
         out["%s_newton_per_step" % name] = [r["newton"] for r in recs]
-        log("# %s: ms per step %s, PCG %s, Newton %s" % (name, ["%.1f" % r["ms"] for r in recs],
-                                                         [r["pcg"] for r in recs], [r["newton"] for r in recs]))
+        if prec == 32:
+            out["%s_fp64_fallback_per_step" % name] = [r["fp64_fallback"] for r in recs]
+        log("# %s: ms per step %s, PCG %s, Newton %s%s" % (
+            name, ["%.1f" % r["ms"] for r in recs], [r["pcg"] for r in recs], [r["newton"] for r in recs],
+            ", FP64 fallback at Newton iteration %s" % [r["fp64_fallback"] for r in recs] if prec == 32 else ""))
     ref = flags[modes[0][0]]
     same = all(np.array_equal(a, b) for m in flags for a, b in zip(ref, flags[m]))
     out["damage_flags_identical"] = bool(same)
     tot = {m[0]: sum(out["%s_ms_per_step" % m[0]]) for m in modes}
     for a, b in (("plain_fp64", "learned_fp32_forcing"), ("plain_fp64_forcing", "learned_fp32_forcing"),
-                 ("plain_fp64_forcing", "learned_fp64_forcing")):
+                 ("plain_fp64", "learned_fp64_forcing"), ("plain_fp64_forcing", "learned_fp64_forcing")):
         if a in tot and b in tot:
             out["speedup_%s_vs_%s" % (b, a)] = tot[a] / tot[b]
             log("# %s vs %s: %.2fx" % (b, a, tot[a] / tot[b]))
@@ -298,6 +302,17 @@ def bench_c2(args):
           flush=True)
     extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather,
              "gather_threads_per_row": slots, "gather_pass_ms_by_threads_per_row": slot_ms}
+    # K_ff SpMV of the Newton solves: FP64 values, and FP32 values with FP64 vectors
+    ctx.assemble(L.PGF_ASM_JAC, dt=0.05)
+    ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FP32, dt=0.05)
+    nnz_ff, nf = ctx.csr.nnz_ff, ctx.csr.n_free
+    for prec, vb in ((64, 12), (32, 8)):
+        t = ctx.time_spmv(prec, reps=20)
+        gbs = (vb * nnz_ff + 4 * (nf + 1) + 16 * nf) / (t * 1e-3) / 1e9
+        extra["spmv_fp%d_ms" % prec] = t
+        extra["spmv_fp%d_GBs" % prec] = gbs
+        print("# C2 SpMV K_ff (%d nonzeros) %s: %.3f ms, %.0f GB/s" % (
+            nnz_ff, "FP64" if prec == 64 else "FP32 values / FP64 vectors", t, gbs), flush=True)
     if not args.no_learn:
         extra.update(learned_vs_plain(args.c2_n, args.c2_incl, args.c2_steps, prob=(prob, incl),
                                       log=lambda m: print(m, flush=True), ctx=ctx, modes=args.c2_modes.split(",")))

@@ -128,7 +128,7 @@ typedef struct pgf_newton_opts {
     double rel_tol;        /* ||r|| < rel_tol*||r0||  (reference rel_tol 1e-8) */
     double abs_tol;        /* or ||r|| < abs_tol */
     int32_t max_iter;      /* Newton iterations (reference max_iter 40) */
-    int32_t precision;     /* 64: FP64 K and PCG; 32: FP32 K and PCG, FP64 residual */
+    int32_t precision;     /* 64: FP64 K and PCG; 32: FP32 K values, FP64 PCG vectors and residual */
     double lin_rtol;       /* PCG relative tolerance per Newton iteration */
     int32_t lin_maxit;
     int32_t use_guess;     /* 1: start PCG from the device guess buffer (pgf_set_guess); 2, 3: learned guess
@@ -136,14 +136,17 @@ typedef struct pgf_newton_opts {
     double dt;             /* time increment passed to the cohesive law */
     int32_t adaptive_lin;  /* 1: per-iteration PCG tolerance max(lin_rtol, 0.5 rel_tol ||r0|| / ||r_k||)
                               (inexact Newton forcing term); 0: lin_rtol every iteration */
-    int32_t reserved;
+    int32_t fp32_maxit;    /* precision 32: PCG iterations allowed on the FP32 matrix before that solve
+                              continues from its iterate on the FP64 matrix, and the rest of the step
+                              runs in FP64 (0: lin_maxit) */
 } pgf_newton_opts;
 
 typedef struct pgf_step_info {
     int32_t newton_iterations;
     int32_t converged;
     int32_t pcg_iterations;     /* summed over the step */
-    int32_t reserved;
+    int32_t fp64_fallback;      /* precision 32: Newton iteration (1-based) whose FP32 solve did not
+                                   converge and was finished in FP64, 0 if none */
     double res0;                /* ||r0||, FP64 */
     double res;                 /* final ||r||, FP64 */
     double ms_total;            /* device time of the step */

@@ -51,12 +51,12 @@ class SolveInfo(C.Structure):
 class NewtonOpts(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("abs_tol", C.c_double), ("max_iter", C.c_int32),
                 ("precision", C.c_int32), ("lin_rtol", C.c_double), ("lin_maxit", C.c_int32),
-                ("use_guess", C.c_int32), ("dt", C.c_double), ("adaptive_lin", C.c_int32), ("reserved", C.c_int32)]
+                ("use_guess", C.c_int32), ("dt", C.c_double), ("adaptive_lin", C.c_int32), ("fp32_maxit", C.c_int32)]
 
 
 class StepInfo(C.Structure):
     _fields_ = [("newton_iterations", C.c_int32), ("converged", C.c_int32), ("pcg_iterations", C.c_int32),
-                ("reserved", C.c_int32), ("res0", C.c_double), ("res", C.c_double), ("ms_total", C.c_double),
+                ("fp64_fallback", C.c_int32), ("res0", C.c_double), ("res", C.c_double), ("ms_total", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double)]
 
 

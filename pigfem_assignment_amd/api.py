@@ -252,9 +252,9 @@ class DeviceContext:
         return x, info
 
     def newton_step(self, frac, rel_tol=1e-8, abs_tol=0.0, max_iter=40, precision=64, lin_rtol=1e-10,
-                    lin_maxit=50000, dt=1.0, use_guess=False, adaptive_lin=False):
+                    lin_maxit=50000, dt=1.0, use_guess=False, adaptive_lin=False, fp32_maxit=0):
         o = L.NewtonOpts(rel_tol, abs_tol, max_iter, precision, lin_rtol, lin_maxit, int(use_guess), dt,
-                         int(adaptive_lin), 0)
+                         int(adaptive_lin), int(fp32_maxit))
         si = L.StepInfo()
         rc = self.lib.pgf_newton_step(self.h, frac, C.byref(o), C.byref(si))
         if rc not in (0, -5):

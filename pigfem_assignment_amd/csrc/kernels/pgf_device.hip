@@ -932,10 +932,10 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
         double res0, res;
         residual(res0);
         res = res0;
-        int it = 0, inc = 0, pcg_its = 0;
+        int it = 0, inc = 0, pcg_its = 0, fallback = 0;
         bool conv = res0 < o->abs_tol || res0 == 0.0, fail = false;
         while (!conv && !fail) {
-            const bool fp32 = o->precision == 32;
+            const bool fp32 = o->precision == 32 && fallback == 0;
             HIPCHK(hipEventRecord(a0, c->stream));
             pgfd::enqueue_assembly(*c, o->dt, PGF_ASM_JAC | (fp32 ? PGF_ASM_FP32 : 0u));
             HIPCHK(hipEventRecord(a1, c->stream));
@@ -953,11 +953,34 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
                 g = pgfd::learn_guess(*c, o->use_guess, fp32, it) || g;
             if (g) hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->guess, c->x);
             // FP32 mode: FP32 tangent values with FP64 Krylov vectors; the
-            // residual, the update and delta_max stay FP64 either way
-            if (fp32)
-                pgfd::run_pcg<float, double>(*c, c->K32, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g,
-                                             lin_tol, o->lin_maxit, &si);
-            else
+            // residual, the update and delta_max stay FP64 either way.  When
+            // the FP32 matrix cannot reach the tolerance (rounding K to FP32
+            // perturbs it by ~6e-8 relative, which a Jacobi-scaled condition
+            // number near 1e7 turns into stagnation), the solve continues
+            // from its iterate on the FP64 matrix and the step stays in FP64.
+            if (fp32) {
+                hipLaunchKernelGGL(pgfd::k_copy, dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf, c->r, c->res);
+                const int maxit32 = o->fp32_maxit > 0 ? std::min(o->fp32_maxit, o->lin_maxit) : o->lin_maxit;
+                pgfd::run_pcg<float, double>(*c, c->K32, c->res, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g,
+                                             lin_tol, maxit32, &si);
+                if (!si.converged) {
+                    fallback = it + 1;
+                    HIPCHK(hipEventRecord(a0, c->stream));
+                    pgfd::enqueue_assembly(*c, o->dt, PGF_ASM_JAC);
+                    HIPCHK(hipEventRecord(a1, c->stream));
+                    HIPCHK(hipEventSynchronize(a1));
+                    HIPCHK(hipEventElapsedTime(&ms, a0, a1));
+                    ms_asm += ms;
+                    const bool warm = std::isfinite(si.rel_residual) && si.rel_residual < 1.0;
+                    pgf_solve_info s64{};
+                    pgfd::run_pcg<double, double>(*c, c->K, c->res, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv,
+                                                  warm, lin_tol, std::max(o->lin_maxit - si.iterations, 1), &s64);
+                    si.iterations += s64.iterations;
+                    si.ms += s64.ms;
+                    si.converged = s64.converged;
+                    si.rel_residual = s64.rel_residual;
+                }
+            } else
                 pgfd::run_pcg<double, double>(*c, c->K, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g,
                                               lin_tol, o->lin_maxit, &si);
             hipLaunchKernelGGL((pgfd::k_update_U<double>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
@@ -982,6 +1005,7 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
             info->newton_iterations = it;
             info->converged = conv ? 1 : 0;
             info->pcg_iterations = pcg_its;
+            info->fp64_fallback = fallback;
             info->res0 = res0;
             info->res = res;
             info->ms_total = mt;

@@ -42,7 +42,7 @@ def cohesive_problem(n, n_incl, seed, law=configs.BILINEAR_C2, r_lo=0.06, r_hi=0
 
 
 def run_steps(prob, incl, n_steps, frac_step, use_guess=0, net=None, record=False, lin_rtol=1e-10, dt=None,
-              precision=64, callback=None, ctx=None, adaptive_lin=False):
+              precision=64, callback=None, ctx=None, adaptive_lin=False, fp32_maxit=0):
     """Solve load steps frac = k * frac_step, k = 1..n_steps from the unloaded
     state; returns per-step records, the damage flags after every step and the
     context (pass it back in to reuse its device set-up)."""
@@ -55,7 +55,8 @@ def run_steps(prob, incl, n_steps, frac_step, use_guess=0, net=None, record=Fals
     for k in range(1, n_steps + 1):
         t0 = time.time()
         si = ctx.newton_step(k * frac_step, dt=dt if dt is not None else frac_step, use_guess=use_guess,
-                             lin_rtol=lin_rtol, precision=precision, adaptive_lin=adaptive_lin)
+                             lin_rtol=lin_rtol, precision=precision, adaptive_lin=adaptive_lin,
+                             fp32_maxit=fp32_maxit)
         wall = time.time() - t0
         if not si.converged:
             raise RuntimeError("load step %d did not converge (mode %d)" % (k, use_guess))
@@ -66,7 +67,7 @@ def run_steps(prob, incl, n_steps, frac_step, use_guess=0, net=None, record=Fals
         g = ctx.learn_last() if use_guess in (2, 3) else None
         rec = dict(step=k, newton=si.newton_iterations, pcg=si.pcg_iterations, ms=si.ms_total,
                    ms_solve=si.ms_solve, ms_asm=si.ms_assemble, res0=si.res0, res=si.res, wall_s=wall,
-                   guess=g)
+                   guess=g, fp64_fallback=si.fp64_fallback)
         out.append(rec)
         flags.append(ctx.isv()[2].copy())
         if callback:

@@ -208,3 +208,27 @@ def test_3d_cohesive_load_steps(precision):
                                                                     si.pcg_iterations, err))
         assert err <= 1e-8 if precision == 64 else err <= 1e-6
         assert np.array_equal(fl_g.astype(np.int32), fl_o)
+
+
+def test_fp32_solve_falls_back_to_fp64():
+    """An FP32 inner solve that cannot reach its tolerance (forced here by a
+    5-iteration FP32 budget) continues on the FP64 matrix from its iterate:
+    the step converges, reports the fallback, and matches the oracle to the
+    FP64 bar with the same damage flags."""
+    p, o = _c2small(configs.BILINEAR_C2)
+    ctx = DeviceContext(p)
+    for s in range(1, 3):
+        frac = 0.05 * s
+        si = ctx.newton_step(frac, dt=0.05, precision=32, lin_rtol=1e-12, fp32_maxit=5)
+        assert si.converged and si.fp64_fallback == 1
+        ctx.isv_commit()
+        o.newton_step(frac, dt=0.05, lin_rtol=1e-14)
+        o.isv_commit()
+        err = np.max(np.abs(ctx.solution() - o.solution())) / np.max(np.abs(o.solution()))
+        print("step %d: newton %d pcg %d fallback at %d, U rel err %.3e" % (
+            s, si.newton_iterations, si.pcg_iterations, si.fp64_fallback, err))
+        assert err <= 1e-8
+        assert np.array_equal(ctx.isv()[2].astype(np.int32), o.isv()[2])
+    # with enough FP32 iterations no fallback happens on this well-conditioned case
+    si = ctx.newton_step(0.15, dt=0.05, precision=32, lin_rtol=1e-5)
+    assert si.converged and si.fp64_fallback == 0
