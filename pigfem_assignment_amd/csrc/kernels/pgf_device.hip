@@ -73,7 +73,7 @@ Ctx::~Ctx() {
                     dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
-                    inc_off, ke, pe, blk_node, blk_seg, blk_rounds};
+                    inc_off, ke, pe, blk_node, blk_seg, blk_rounds, grp_row};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -185,27 +185,50 @@ enum { S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_RR0, S_TOL2, S_ITER, S_MAXIT, S_BB, S_
 constexpr int RED_BLOCKS = 1024;
 
 // ---------------------------------------------------------------------------
-// SpMV on the free-free block: y = K_ff x, G lanes per row.
+// SpMV on the free-free block, y = K_ff x, by node groups: the free rows of
+// one node are consecutive and have the same free column list, so G lanes
+// serve a group of up to three rows together and every column index and x
+// entry is loaded once for all of them (the column index stream shrinks to a
+// third in 3D).  Each row still sums its entries lane-strided and then across
+// the G lanes, the same order as one row per G lanes.
 template <typename TA, typename T, int G>
-__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int32_t* __restrict__ rowptr,
-                                              const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
-                                              const TA* __restrict__ A, const T* __restrict__ x, T* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_spmv(int64_t ng, const int32_t* __restrict__ grow,
+                                              const int32_t* __restrict__ rowptr, const int32_t* __restrict__ rowsplit,
+                                              const int32_t* __restrict__ col, const TA* __restrict__ A,
+                                              const T* __restrict__ x, T* __restrict__ y) {
     const int lane = threadIdx.x % G;
-    for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; row < n;
-         row += (int64_t)gridDim.x * blockDim.x / G) {
-        const int32_t b = rowptr[row], e = b + rowsplit[row];
-        double acc = 0.0;
-        for (int32_t j = b + lane; j < e; j += G) acc += (double)A[j] * (double)x[col[j]];
-        for (int m = G / 2; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, G);
-        if (lane == 0) y[row] = (T)acc;
+    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; g < ng;
+         g += (int64_t)gridDim.x * blockDim.x / G) {
+        const int32_t r0 = grow[g], k = grow[g + 1] - r0;
+        const int32_t b0 = rowptr[r0], len = rowsplit[r0];
+        const int32_t b1 = k > 1 ? rowptr[r0 + 1] : b0, b2 = k > 2 ? rowptr[r0 + 2] : b0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int32_t j = lane; j < len; j += G) {
+            const double xv = (double)x[col[b0 + j]];
+            a0 += (double)A[b0 + j] * xv;
+            if (k > 1) a1 += (double)A[b1 + j] * xv;
+            if (k > 2) a2 += (double)A[b2 + j] * xv;
+        }
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            a0 += __shfl_xor(a0, m, G);
+            a1 += __shfl_xor(a1, m, G);
+            a2 += __shfl_xor(a2, m, G);
+        }
+        if (lane == 0) {
+            y[r0] = (T)a0;
+            if (k > 1) y[r0 + 1] = (T)a1;
+            if (k > 2) y[r0 + 2] = (T)a2;
+        }
     }
 }
 
-// PCG step A: p_new = z + beta p_old (own rows), q = A p_new (rows gathered on
-// the fly from z and p_old), alpha = rz / (p.q) in the last block.  The
-// matrix may be stored in a narrower type than the Krylov vectors.
+// PCG step A: p_new = z + beta p_old (own rows), q = A p_new (columns gathered
+// on the fly from z and p_old, once per node group), alpha = rz / (p.q) in the
+// last block.  The matrix may be stored in a narrower type than the Krylov
+// vectors.
 template <typename TA, typename T, int G>
-__global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int32_t* __restrict__ rowptr,
+__global__ __launch_bounds__(256) void k_cg_spmv(int64_t ng, const int32_t* __restrict__ grow,
+                                                 const int32_t* __restrict__ rowptr,
                                                  const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
                                                  const TA* __restrict__ A, const T* __restrict__ z,
                                                  const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
@@ -214,20 +237,32 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int32_t* __res
     const double beta = scal[S_BETA];
     const int lane = threadIdx.x % G;
     double dot = 0.0;
-    for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; row < n;
-         row += (int64_t)gridDim.x * blockDim.x / G) {
-        const int32_t b = rowptr[row], e = b + rowsplit[row];
-        double acc = 0.0;
-        for (int32_t j = b + lane; j < e; j += G) {
-            const int32_t c = col[j];
-            acc += (double)A[j] * ((double)z[c] + beta * (double)pold[c]);
+    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; g < ng;
+         g += (int64_t)gridDim.x * blockDim.x / G) {
+        const int32_t r0 = grow[g], k = grow[g + 1] - r0;
+        const int32_t b0 = rowptr[r0], len = rowsplit[r0];
+        const int32_t b1 = k > 1 ? rowptr[r0 + 1] : b0, b2 = k > 2 ? rowptr[r0 + 2] : b0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int32_t j = lane; j < len; j += G) {
+            const int32_t c = col[b0 + j];
+            const double pv = (double)z[c] + beta * (double)pold[c];
+            a0 += (double)A[b0 + j] * pv;
+            if (k > 1) a1 += (double)A[b1 + j] * pv;
+            if (k > 2) a2 += (double)A[b2 + j] * pv;
         }
-        for (int m = G / 2; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, G);
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            a0 += __shfl_xor(a0, m, G);
+            a1 += __shfl_xor(a1, m, G);
+            a2 += __shfl_xor(a2, m, G);
+        }
         if (lane == 0) {
-            const double pn = (double)z[row] + beta * (double)pold[row];
-            pnew[row] = (T)pn;
-            q[row] = (T)acc;
-            dot += pn * acc;
+            const double av[3] = {a0, a1, a2};
+            for (int t = 0; t < k; ++t) {
+                const double pn = (double)z[r0 + t] + beta * (double)pold[r0 + t];
+                pnew[r0 + t] = (T)pn;
+                q[r0 + t] = (T)av[t];
+                dot += pn * av[t];
+            }
         }
     }
     double v[1] = {dot};
@@ -343,9 +378,13 @@ static unsigned grid_for(int64_t n, int per_block = 256) {
 
 template <typename TA, typename T>
 static void enqueue_spmv(Ctx& c, const TA* A, const T* x, T* y) {
-    const unsigned nb = (unsigned)std::min<int64_t>((c.n_free * 16 + 255) / 256, 65536);
-    if (c.dim == 3) hipLaunchKernelGGL((k_spmv<TA, T, 16>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
-    else hipLaunchKernelGGL((k_spmv<TA, T, 8>), dim3(nb), dim3(256), 0, c.stream, c.n_free, c.rowptr, c.rowsplit, c.colidx, A, x, y);
+    const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((c.n_grp * 16 + 255) / 256, 65536));
+    if (c.dim == 3)
+        hipLaunchKernelGGL((k_spmv<TA, T, 16>), dim3(nb), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr, c.rowsplit,
+                           c.colidx, A, x, y);
+    else
+        hipLaunchKernelGGL((k_spmv<TA, T, 8>), dim3(nb), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr, c.rowsplit,
+                           c.colidx, A, x, y);
 }
 
 // Enqueue a whole PCG solve in chunks; the device flag stops the recurrence,
@@ -375,7 +414,7 @@ static int run_pcg(Ctx& c, const TA* A, const void* b, bool b_is_double, T* x, T
                            x0 ? (const T*)q : nullptr, (const T*)dinv, c.scal, c.partial, c.counter, c.done, rtol * rtol,
                            (double)maxit);
     const int G = c.dim == 3 ? 16 : 8;
-    const unsigned ga = (unsigned)std::min<int64_t>((n * G + 255) / 256, RED_BLOCKS * 2);
+    const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>((c.n_grp * G + 255) / 256, RED_BLOCKS * 2));
     const unsigned gu = grid_for(n);
     const int chunk = 32;
     int it = 0;
@@ -388,11 +427,11 @@ static int run_pcg(Ctx& c, const TA* A, const void* b, bool b_is_double, T* x, T
             T* pold = (it % 2 == 0) ? p0 : p1;
             T* pnew = (it % 2 == 0) ? p1 : p0;
             if (G == 16)
-                hipLaunchKernelGGL((k_cg_spmv<TA, T, 16>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
-                                   A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
+                hipLaunchKernelGGL((k_cg_spmv<TA, T, 16>), dim3(ga), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr,
+                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
             else
-                hipLaunchKernelGGL((k_cg_spmv<TA, T, 8>), dim3(ga), dim3(256), 0, c.stream, n, c.rowptr, c.rowsplit, c.colidx,
-                                   A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
+                hipLaunchKernelGGL((k_cg_spmv<TA, T, 8>), dim3(ga), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr,
+                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
             hipLaunchKernelGGL((k_cg_update<T>), dim3(gu), dim3(256), 0, c.stream, n, x, r, z, pnew, q, dinv, c.scal,
                                c.partial, c.counter, c.done, (double*)nullptr);
         }
@@ -679,6 +718,22 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         for (int64_t r = 0; r <= c->n_rows; ++r) rp32[r] = (int32_t)pat.rowptr[r];
         c->rowptr = pgfd::dupload(rp32.data(), c->n_rows + 1);
         c->rowsplit = pgfd::dupload(pat.rowsplit.data(), c->n_rows);
+        // node groups of the free rows for the SpMV: runs of at most three
+        // consecutive rows with the same free column list
+        {
+            std::vector<int32_t> grp{0};
+            for (int64_t r = 1; r < c->n_free; ++r) {
+                const int64_t r0 = grp.back();
+                bool same = r - r0 < 3 && pat.rowsplit[r] == pat.rowsplit[r0];
+                if (same)
+                    same = std::equal(pat.colidx.begin() + pat.rowptr[r], pat.colidx.begin() + pat.rowptr[r] + pat.rowsplit[r],
+                                      pat.colidx.begin() + pat.rowptr[r0]);
+                if (!same) grp.push_back((int32_t)r);
+            }
+            if (c->n_free > 0) grp.push_back((int32_t)c->n_free);
+            c->n_grp = c->n_free > 0 ? (int64_t)grp.size() - 1 : 0;
+            c->grp_row = pgfd::dupload(grp.data(), (int64_t)grp.size());
+        }
         c->colidx = pgfd::dupload(pat.colidx.data(), c->nnz);
         c->K = pgfd::dalloc<double>(c->nnz);
         c->K32 = pgfd::dalloc<float>(c->nnz);

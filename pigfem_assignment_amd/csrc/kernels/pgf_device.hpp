@@ -58,6 +58,8 @@ struct Ctx {
     int64_t *free_slot = nullptr, *pres_slot = nullptr;
     // matrix
     int32_t *rowptr = nullptr, *rowsplit = nullptr, *colidx = nullptr;
+    int32_t* grp_row = nullptr;  // [n_grp+1] first free row of each SpMV node group
+    int64_t n_grp = 0;
     double* K = nullptr;
     float* K32 = nullptr;
     double* P = nullptr;
