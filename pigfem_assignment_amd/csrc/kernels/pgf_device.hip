@@ -125,6 +125,26 @@ __device__ inline bool last_block(double* v, double* partial, unsigned* counter)
     return true;
 }
 
+// One block's sums to partial[k * gridDim.x + blockIdx.x]; the next kernel on
+// the stream reads them.
+template <int NV>
+__device__ inline void block_partial(double* v, double* partial) {
+    block_reduce<NV>(v);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) partial[k * gridDim.x + blockIdx.x] = v[k];
+}
+
+// Fixed-order sum of nb partial sums of NV values by one block.
+template <int NV>
+__device__ inline void sum_partials(int nb, const double* partial, double* v) {
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[k * nb + i];
+        v[k] = s;
+    }
+    block_reduce<NV>(v);
+}
+
 // Gather blocks: runs of consecutive nodes, `slots` threads per matrix row,
 // GATHER_THREADS / (slots * dim) nodes per block.  A block's free rows are
 // staged in LDS when they form one contiguous CSR segment of at most SEG_MAX
@@ -183,6 +203,13 @@ void build_gather_blocks(Ctx& c, int slots) {
 enum { S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_RR0, S_TOL2, S_ITER, S_MAXIT, S_BB, S_NSCAL = 16 };
 
 constexpr int RED_BLOCKS = 1024;
+// PCG iteration kernels write one partial sum per block and a one-block kernel
+// sums them in a fixed order: no atomics or fences in the iteration.  A single
+// completion counter per kernel costs ~60 ns per block on one L2 address,
+// which at 2,000-30,000 blocks was most of an iteration.
+constexpr int CG_SPMV_BLOCKS = 65536;   // cap of the SpMV grid (partials slots)
+constexpr int CG_UPD_BLOCKS = 2048;     // grid of the vector update
+constexpr int CG_PART = CG_SPMV_BLOCKS + 2 * CG_UPD_BLOCKS;  // partial slots of one context
 
 // ---------------------------------------------------------------------------
 // SpMV on the free-free block, y = K_ff x, by node groups: the free rows of
@@ -232,7 +259,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t ng, const int32_t* __re
                                                  const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
                                                  const TA* __restrict__ A, const T* __restrict__ z,
                                                  const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
-                                                 double* scal, double* partial, unsigned* counter, const int* done) {
+                                                 const double* scal, double* partial, const int* done) {
     if (*done) return;
     const double beta = scal[S_BETA];
     const int lane = threadIdx.x % G;
@@ -266,10 +293,18 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t ng, const int32_t* __re
         }
     }
     double v[1] = {dot};
-    if (last_block<1>(v, partial, counter) && threadIdx.x == 0) {
+    block_partial<1>(v, partial);
+}
+
+// alpha = rz / (p.q) from the SpMV blocks' partials.
+__global__ __launch_bounds__(1024) void k_cg_alpha(int nb, const double* __restrict__ partial, double* scal,
+                                                   const int* done) {
+    if (*done) return;
+    double v[1];
+    sum_partials<1>(nb, partial, v);
+    if (threadIdx.x == 0) {
         const double pq = v[0];
         scal[S_ALPHA] = (pq != 0.0 && std::isfinite(pq)) ? scal[S_RZ] / pq : 0.0;
-        *counter = 0;
     }
 }
 
@@ -278,8 +313,8 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t ng, const int32_t* __re
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_update(int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                    const T* __restrict__ p, const T* __restrict__ q,
-                                                   const T* __restrict__ dinv, double* scal, double* partial,
-                                                   unsigned* counter, int* done, double* hist) {
+                                                   const T* __restrict__ dinv, const double* scal, double* partial,
+                                                   const int* done) {
     if (*done) return;
     const double alpha = scal[S_ALPHA];
     double rz = 0.0, rr = 0.0;
@@ -292,16 +327,23 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, T* __restrict__ x,
         rr += ri * ri;
     }
     double v[2] = {rz, rr};
-    if (last_block<2>(v, partial, counter + 1) && threadIdx.x == 0) {
+    block_partial<2>(v, partial);
+}
+
+// beta = rz_new / rz_old, the iteration count and the convergence flag from
+// the update blocks' partials.
+__global__ __launch_bounds__(1024) void k_cg_beta(int nb, const double* __restrict__ partial, double* scal, int* done) {
+    if (*done) return;
+    double v[2];
+    sum_partials<2>(nb, partial, v);
+    if (threadIdx.x == 0) {
         const double rz_old = scal[S_RZ];
         scal[S_BETA] = rz_old != 0.0 ? v[0] / rz_old : 0.0;
         scal[S_RZ] = v[0];
         scal[S_RR] = v[1];
         const double it = scal[S_ITER] + 1.0;
         scal[S_ITER] = it;
-        if (hist) hist[(int)it] = v[1];
         if (!(v[1] > scal[S_TOL2] * scal[S_RR0]) || it >= scal[S_MAXIT] || !std::isfinite(v[1])) *done = 1;
-        counter[1] = 0;
     }
 }
 
@@ -414,8 +456,10 @@ static int run_pcg(Ctx& c, const TA* A, const void* b, bool b_is_double, T* x, T
                            x0 ? (const T*)q : nullptr, (const T*)dinv, c.scal, c.partial, c.counter, c.done, rtol * rtol,
                            (double)maxit);
     const int G = c.dim == 3 ? 16 : 8;
-    const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>((c.n_grp * G + 255) / 256, RED_BLOCKS * 2));
-    const unsigned gu = grid_for(n);
+    const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>((c.n_grp * G + 255) / 256, CG_SPMV_BLOCKS));
+    const unsigned gu = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, CG_UPD_BLOCKS));
+    double* part_a = c.partial;                   // [ga] p.q
+    double* part_u = c.partial + CG_SPMV_BLOCKS;  // [2][gu] rz, rr
     const int chunk = 32;
     int it = 0;
     bool stop = false;
@@ -428,12 +472,17 @@ static int run_pcg(Ctx& c, const TA* A, const void* b, bool b_is_double, T* x, T
             T* pnew = (it % 2 == 0) ? p1 : p0;
             if (G == 16)
                 hipLaunchKernelGGL((k_cg_spmv<TA, T, 16>), dim3(ga), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr,
-                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
+                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, (const double*)c.scal, part_a,
+                                   (const int*)c.done);
             else
                 hipLaunchKernelGGL((k_cg_spmv<TA, T, 8>), dim3(ga), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr,
-                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, c.scal, c.partial, c.counter, c.done);
-            hipLaunchKernelGGL((k_cg_update<T>), dim3(gu), dim3(256), 0, c.stream, n, x, r, z, pnew, q, dinv, c.scal,
-                               c.partial, c.counter, c.done, (double*)nullptr);
+                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, (const double*)c.scal, part_a,
+                                   (const int*)c.done);
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c.stream, (int)ga, (const double*)part_a, c.scal,
+                               (const int*)c.done);
+            hipLaunchKernelGGL((k_cg_update<T>), dim3(gu), dim3(256), 0, c.stream, n, x, r, z, pnew, q, dinv,
+                               (const double*)c.scal, part_u, (const int*)c.done);
+            hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, c.stream, (int)gu, (const double*)part_u, c.scal, c.done);
         }
         HIPCHK(hipMemcpyAsync(c.h_scal, c.scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipMemcpyAsync(c.h_scal + S_NSCAL, c.done, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -746,7 +795,7 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         c->r = pgfd::dalloc<double>(nf); c->x = pgfd::dalloc<double>(nf); c->z = pgfd::dalloc<double>(nf);
         c->p0 = pgfd::dalloc<double>(nf); c->p1 = pgfd::dalloc<double>(nf); c->q = pgfd::dalloc<double>(nf);
         c->dinv = pgfd::dalloc<double>(nf); c->guess = pgfd::dalloc<double>(nf); c->res = pgfd::dalloc<double>(nf);
-        c->partial = pgfd::dalloc<double>(3 * 2 * pgfd::RED_BLOCKS);
+        c->partial = pgfd::dalloc<double>(std::max(3 * 2 * pgfd::RED_BLOCKS, pgfd::CG_PART));
         c->scal = pgfd::dalloc<double>(pgfd::S_NSCAL + 8);
         c->counter = pgfd::dalloc<unsigned>(8);
         c->done = pgfd::dalloc<int>(1);
