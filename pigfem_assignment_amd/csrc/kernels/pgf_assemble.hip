@@ -267,18 +267,27 @@ __global__ __launch_bounds__(256) void k_uncut(AsmArgs a, int64_t begin, int64_t
 }
 
 // ---------------------------------------------------------------------------
-// Cut element computed by one wavefront into LDS: K_e (M x M, row stride M + 1)
-// and P_e (M).
+// Cut element computed by one wavefront: K_e (M x M) entries handed to
+// emit(i, j, value), P_e (M) into LDS.
 // Every integration point is staged in LDS first (gradients, J*W, the point's
 // material, its stress; per cohesive point the rotated tangent R^T D_c R, the
 // rotated traction and the surface shape functions), then each lane owns one
-// row of K_e (or a column group of it), forms that row's D B column once per
-// point and sums every point's term into its entries in point order
-// (volumetric points, then cohesive points): three barriers per element.  The cohesive law runs
-// once per cohesive point (lane q), which also writes that point's trial
-// delta_max and damage flags.
-template <int DIM, bool DO_K, bool DO_F>
-__device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K, double* s_P, int* s_node, int* s_dof) {
+// row of K_e (or a column group of it: entries j = grp + t * nsplit), forms
+// that row's D B column once per point and sums every point's term into its
+// entries in point order (volumetric points, then cohesive points): three
+// barriers per element.  In 2D (M <= 24, at most 12 entries per lane, no
+// cohesive points on the path's 2D workloads) the entries live in registers,
+// so no LDS tile for K_e limits how many elements share a CU; in 3D (up to 36
+// entries per lane, cohesive points matched per entry) they live in an LDS
+// row of stride M + 1.  The cohesive law runs once per cohesive point (lane
+// q), which also writes that point's trial delta_max and damage flags.
+template <int DIM>
+__host__ __device__ constexpr int cut_row_entries() {
+    return (MAX_NT * DIM + cut_threads<DIM>() / (MAX_NT * DIM) - 1) / (cut_threads<DIM>() / (MAX_NT * DIM));
+}
+
+template <int DIM, bool DO_K, bool DO_F, typename Emit>
+__device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_P, int* s_node, int* s_dof, Emit emit) {
     const int tid = threadIdx.x;
     constexpr int S = DIM == 2 ? 3 : 6;
     constexpr int MM = MAX_NT * DIM;
@@ -297,11 +306,11 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
     const int nc = (int)(a.cpt_ptr[c + 1] - c0);
     if (tid < nt) s_node[tid] = a.ce_nodes[c * 16 + tid];
     __syncthreads();
-    for (int i = tid; i < M; i += CUT_THREADS) {
+    for (int i = tid; i < M; i += cut_threads<DIM>()) {
         s_dof[i] = a.dof[(i % DIM) * a.n_all + s_node[i / DIM]];
         s_u[i] = a.U[(i % DIM) * a.n_all + s_node[i / DIM]];
     }
-    for (int k = tid; k < nv * M; k += CUT_THREADS) {
+    for (int k = tid; k < nv * M; k += cut_threads<DIM>()) {
         const int q = k / M;
         s_g[q * MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
     }
@@ -356,14 +365,63 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
         s_cw[tid] = a.cpt_w[q];
     }
     __syncthreads();
-    if (DO_K) {
+    if (DO_K && DIM == 2) {
         // lane -> (row i, column group grp): the lane owns K_e(i, j) for
-        // j = grp mod nsplit and adds every point's term to them in order
-        const int nsplit = CUT_THREADS / M;
+        // j = grp + t * nsplit, held in registers
+        constexpr int TMAX = cut_row_entries<DIM>();
+        const int nsplit = cut_threads<DIM>() / M;
         if (tid < M * nsplit) {
             const int i = tid % M, grp = tid / M;
             const int an = i / DIM, da = i % DIM;
-            double* Ki = s_K + i * (M + 1);  // row stride M + 1: lanes of different rows hit different banks
+            double acc[TMAX];
+#pragma unroll
+            for (int t = 0; t < TMAX; ++t) acc[t] = 0.0;
+            for (int q = 0; q < nv; ++q) {
+                double parts[S];
+                k_parts_col<DIM>(&s_g[q * MM + an * DIM], da, iso_of<DIM>(a.Dmat + s_vmat[q] * 36), parts);
+                const double wq = s_w[q];
+                const double* gq = &s_g[q * MM];
+#pragma unroll
+                for (int t = 0; t < TMAX; ++t) {
+                    const int j = grp + t * nsplit;
+                    if (j < M) acc[t] += wq * k_entry<DIM>(parts, gq + (j / DIM) * DIM, j % DIM);
+                }
+            }
+            for (int q = 0; q < nc; ++q) {
+                const int nh = s_cnh[q];
+                for (int k1 = 0; k1 < 2 * nh; ++k1) {
+                    if (s_cmap[q * 8 + k1] != an) continue;
+                    const double n1 = s_cN[q * 4 + k1 % nh];
+#pragma unroll
+                    for (int t = 0; t < TMAX; ++t) {
+                        const int j = grp + t * nsplit;
+                        if (j >= M) continue;
+                        const int bn = j / DIM, db = j % DIM;
+                        for (int k2 = 0; k2 < 2 * nh; ++k2) {
+                            if (s_cmap[q * 8 + k2] != bn) continue;
+                            const double sgn = ((k1 < nh) == (k2 < nh)) ? 1.0 : -1.0;
+                            const double v = n1 * s_cN[q * 4 + k2 % nh] * s_cRDR[q * DIM * DIM + da * DIM + db];
+                            acc[t] += s_cw[q] * (sgn * v);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < TMAX; ++t) {
+                const int j = grp + t * nsplit;
+                if (j < M) emit(i, j, acc[t]);
+            }
+        }
+    }
+    if (DO_K && DIM == 3) {
+        // lane -> (row i, column group grp): the lane's entries of row i live
+        // in LDS (row stride M + 1: lanes of different rows hit different banks)
+        __shared__ double s_K[MM * (MM + 1)];
+        const int nsplit = cut_threads<DIM>() / M;
+        if (tid < M * nsplit) {
+            const int i = tid % M, grp = tid / M;
+            const int an = i / DIM, da = i % DIM;
+            double* Ki = s_K + i * (M + 1);
             for (int j = grp; j < M; j += nsplit) Ki[j] = 0.0;
             for (int q = 0; q < nv; ++q) {
                 double parts[S];
@@ -388,10 +446,11 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
                     }
                 }
             }
+            for (int j = grp; j < M; j += nsplit) emit(i, j, Ki[j]);
         }
     }
     if (DO_F)
-        for (int i = tid; i < M; i += CUT_THREADS) {
+        for (int i = tid; i < M; i += cut_threads<DIM>()) {
             const int an = i / DIM, d = i % DIM;
             double acc = 0.0;
             #pragma unroll 1
@@ -413,56 +472,50 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_K,
 
 // Coloured scheme, cut elements: compute then scatter into the CSR rows.
 template <int DIM, bool DO_K, bool DO_F, typename KT>
-__global__ __launch_bounds__(CUT_THREADS) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
+__global__ __launch_bounds__(cut_threads<DIM>()) void k_cut(AsmArgs a, int64_t begin, int64_t end) {
     const int64_t li = begin + blockIdx.x;
     if (li >= end) return;
     constexpr int MM = MAX_NT * DIM;
     const int c = a.ce_list[li];
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
-    __shared__ double s_K[MM * (MM + 1)];
     __shared__ double s_P[MM];
     __shared__ int s_node[MAX_NT], s_dof[MM];
-    cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
     const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
+    // entries scattered straight from the lanes' registers (one colour: no
+    // other element touches these rows' entries meanwhile)
+    cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_P, s_node, s_dof, [&](int i, int j, double v) {
+        const int r = s_dof[i];
+        if (r >= a.n_free && !a.all_rows) return;
+        const int an = i / DIM, bn = j / DIM, db = j % DIM;
+        const int of = off[(an * nt + bn) * 2], op = off[(an * nt + bn) * 2 + 1];
+        int cf = 0, cp = 0;
+        for (int k = 0; k < db; ++k) { if (s_dof[bn * DIM + k] < a.n_free) ++cf; else ++cp; }
+        const int cdof = s_dof[j];
+        const int64_t pos = cdof < a.n_free ? a.rowptr[r] + of + cf : a.rowptr[r] + op + cp;
+        kadd<KT>(a, pos, v);
+    });
     if (DO_F)
-        for (int i = tid; i < M; i += CUT_THREADS) {
+        for (int i = tid; i < M; i += cut_threads<DIM>()) {
             const int r = s_dof[i];
             if (r < a.n_free || a.all_rows) a.P[r] += s_P[i];
-        }
-    if (DO_K)
-        for (int e = tid; e < M * M; e += CUT_THREADS) {
-            const int i = e / M, j = e % M;
-            const int r = s_dof[i];
-            if (r >= a.n_free && !a.all_rows) continue;
-            const int an = i / DIM, bn = j / DIM, db = j % DIM;
-            const int of = off[(an * nt + bn) * 2], op = off[(an * nt + bn) * 2 + 1];
-            int cf = 0, cp = 0;
-            for (int k = 0; k < db; ++k) { if (s_dof[bn * DIM + k] < a.n_free) ++cf; else ++cp; }
-            const int cdof = s_dof[j];
-            const int64_t pos = cdof < a.n_free ? a.rowptr[r] + of + cf : a.rowptr[r] + op + cp;
-            kadd<KT>(a, pos, s_K[i * (M + 1) + j]);
         }
 }
 
 // Gather scheme, pass 1: every cut element's K_e / P_e into the dense scratch.
 template <int DIM, bool DO_K, bool DO_F>
-__global__ __launch_bounds__(CUT_THREADS) void k_cut_elem(AsmArgs a) {
+__global__ __launch_bounds__(cut_threads<DIM>()) void k_cut_elem(AsmArgs a) {
     constexpr int MM = MAX_NT * DIM;
     const int64_t cl = xcd_logical(blockIdx.x, gridDim.x);
     if (cl >= a.n_cut) return;  // padding block of the XCD mapping
     const int c = (int)cl;
     const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
-    __shared__ double s_K[MM * (MM + 1)];
     __shared__ double s_P[MM];
     __shared__ int s_node[MAX_NT], s_dof[MM];
-    cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_K, s_P, s_node, s_dof);
-    if (DO_K) {
-        double* ke = a.ke + a.ke_ptr[c];
-        for (int e = tid; e < M * M; e += CUT_THREADS) ke[e] = s_K[(e / M) * (M + 1) + e % M];
-    }
+    double* ke = a.ke + a.ke_ptr[c];
+    cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_P, s_node, s_dof, [&](int i, int j, double v) { ke[i * M + j] = v; });
     if (DO_F) {
         double* pe = a.pe + a.pe_ptr[c];
-        for (int i = tid; i < M; i += CUT_THREADS) pe[i] = s_P[i];
+        for (int i = tid; i < M; i += cut_threads<DIM>()) pe[i] = s_P[i];
     }
 }
 
@@ -627,7 +680,7 @@ static void launch_colors(const Ctx& c, const pgfk::AsmArgs& a) {
         }
         int64_t cb = c.ce_color_ptr[col], ce = c.ce_color_ptr[col + 1];
         if (ce > cb)
-            hipLaunchKernelGGL((pgfk::k_cut<DIM, DO_K, DO_F, KT>), dim3((unsigned)(ce - cb)), dim3(pgfk::CUT_THREADS), 0, c.stream, a,
+            hipLaunchKernelGGL((pgfk::k_cut<DIM, DO_K, DO_F, KT>), dim3((unsigned)(ce - cb)), dim3(pgfk::cut_threads<DIM>()), 0, c.stream, a,
                                cb, ce);
     }
 }
@@ -637,7 +690,7 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     if (c.timing_events) (void)hipEventRecord(c.timing_events[0], c.stream);
     if (c.n_cut > 0)
         hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(c.n_cut)),
-                           dim3(pgfk::CUT_THREADS), 0, c.stream, a);
+                           dim3(pgfk::cut_threads<DIM>()), 0, c.stream, a);
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
     const dim3 grid((unsigned)pgfk::xcd_blocks(c.n_gblk)), blk(pgfk::GATHER_THREADS);

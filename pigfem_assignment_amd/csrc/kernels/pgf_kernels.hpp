@@ -24,7 +24,8 @@ constexpr int MAX_M = MAX_NT * 3;   // element dofs
 constexpr int max_vpt(int dim) { return dim == 2 ? 32 : 16; }
 constexpr int MAX_CPT = 8;
 // threads of the cut-element kernels (one workgroup per cut element)
-constexpr int CUT_THREADS = 64;
+template <int DIM>
+__host__ __device__ constexpr int cut_threads() { return 64; }
 // row-gather blocks: threads (slots x matrix rows) and LDS segment (doubles)
 constexpr int GATHER_THREADS = 256;
 constexpr int SEG_MAX = 4096;
