@@ -41,18 +41,20 @@ def gather_bytes(dim, nnz_rows, n_rows, n_all, n_elem):
     return 8 * nnz_rows + 8 * n_rows + 2 * 8 * dim * n_all + 17 * n_elem
 
 
-def tune_gather_slots(ctx, flags, reps=5, log=print):
-    """Time the gather pass at 1, 2, 4 and 8 threads per matrix row (results
-    are bit-identical) and keep the fastest; untimed set-up, like an autotuner."""
+def tune_gather_layout(ctx, flags, reps=5, log=print):
+    """Time the gather pass for every layout, 1, 2, 4 or 8 threads per unit
+    and a unit being one matrix row or all rows of a node (results are
+    bit-identical), and keep the fastest; untimed set-up, like an autotuner."""
     ms = {}
-    for slots in (1, 2, 4, 8):
-        ctx.gather_slots = slots
-        ctx.assemble(flags)
-        ms[slots] = ctx.time_assemble_split(flags, reps=reps)[2]
+    for node_rows in (0, 1):
+        for slots in (1, 2, 4, 8):
+            ctx.gather_layout = (slots, node_rows)
+            ctx.assemble(flags)
+            ms["%s%d" % ("node" if node_rows else "row", slots)] = ctx.time_assemble_split(flags, reps=reps)[2]
     best = min(ms, key=ms.get)
-    ctx.gather_slots = best
-    log("# gather pass ms by threads per row: %s -> %d" % (
-        ", ".join("%d: %.3f" % (k, v) for k, v in ms.items()), best))
+    ctx.gather_layout = (int(best[-1]), 1 if best.startswith("node") else 0)
+    log("# gather pass ms by layout (threads per row / per node): %s -> %s" % (
+        ", ".join("%s: %.3f" % (k, v) for k, v in ms.items()), best))
     return best, ms
 
 
@@ -102,7 +104,7 @@ def main():
     print("# device context ready: nnz %d, %d colours" % (ctx.csr.nnz, ctx.csr.n_colors), flush=True)
 
     flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
-    slots, slot_ms = tune_gather_slots(ctx, flags, log=lambda m: print(m, flush=True))
+    slots, slot_ms = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags)
     ms = ctx.time_assemble(flags, reps=args.steps)
@@ -121,7 +123,7 @@ def main():
     print("# assembly K+f FP64 (coloured RMW): %.3f ms/step, %.1f GB/s algorithmic (RMW bytes)" % (
         ms_col, info.n_elem * algorithmic_bytes_per_elem(2) / (ms_col * 1e-3) / 1e9), flush=True)
 
-    extra = {"gather_threads_per_row": slots, "gather_pass_ms_by_threads_per_row": slot_ms,
+    extra = {"gather_layout": slots, "gather_pass_ms_by_layout": slot_ms,
              "assembly_gather_ms": ms, "assembly_gather_cut_pass_ms": ms_cut,
              "assembly_gather_pass_ms": ms_gather, "assembly_gather_bytes": bytes_gather,
              "assembly_colored_ms": ms_col,
@@ -290,7 +292,7 @@ def bench_c2(args):
         info.n_elem, info.n_cut, info.n_free, info.n_cpt, time.time() - t0), flush=True)
     ctx = DeviceContext(prob)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
-    slots, slot_ms = tune_gather_slots(ctx, flags, log=lambda m: print(m, flush=True))
+    slots, slot_ms = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags, dt=0.05)
     ms = ctx.time_assemble(flags, reps=args.steps)
@@ -301,7 +303,7 @@ def bench_c2(args):
     print("# C2 assembly K+f: %.2f ms (cut pass %.2f, gather %.2f), %.1f GB/s" % (ms, ms_cut, ms_gather, achieved),
           flush=True)
     extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather,
-             "gather_threads_per_row": slots, "gather_pass_ms_by_threads_per_row": slot_ms}
+             "gather_layout": slots, "gather_pass_ms_by_layout": slot_ms}
     # K_ff SpMV of the Newton solves: FP64 values, and FP32 values with FP64 vectors
     ctx.assemble(L.PGF_ASM_JAC, dt=0.05)
     ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FP32, dt=0.05)
@@ -344,7 +346,7 @@ def bench_c3(args):
         len(parts), B.n_elem, B.n_cut, B.n_free, time.time() - t0), flush=True)
     ctx = DeviceContext(B)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS
-    slots, _ = tune_gather_slots(ctx, flags, log=lambda m: print(m, flush=True))
+    slots, _ = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags)
     ms_asm = ctx.time_assemble(flags, reps=args.steps)
@@ -367,7 +369,7 @@ def bench_c3(args):
                    "elements": B.n_elem, "dofs": B.n_free},
         "roofline": None, "cpu_baseline": None,
         "extra": {"batched_solve_ms": ms_solve, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
-                  "iterations_total": int(its.sum()), "autotuned": list(tune), "gather_threads_per_row": slots,
+                  "iterations_total": int(its.sum()), "autotuned": list(tune), "gather_layout": slots,
                   "C_eff_mean": Ce.mean(axis=0).tolist()},
     }
     print(json.dumps(line))

@@ -231,10 +231,12 @@ PGF_API int pgf_time_spmv(pgf_ctx* ctx, int32_t precision, int32_t reps, double*
 
 /* ---- tuning of the row-gather assembly (no counterpart in the reference:
  * the reference assembles with MatSetValues, src/Assembler.cpp:47-184) ----
- * threads per matrix row of the gather pass: 1, 2, 4 or 8 (default 1 in 2D,
- * 4 in 3D); rebuilds the node-block partition, results do not change */
-PGF_API int pgf_set_gather_slots(pgf_ctx* ctx, int32_t slots);
-PGF_API int pgf_get_gather_slots(pgf_ctx* ctx, int32_t* slots);
+ * slots: threads that share one unit's incident elements, 1, 2, 4 or 8;
+ * node_rows: 0 = a unit is one matrix row, 1 = a unit is all rows of a node.
+ * Default: 1 slot, one row per unit in 2D; 2 slots in 3D.  Rebuilds the
+ * node-block partition; the assembled values do not change (bit for bit). */
+PGF_API int pgf_set_gather_layout(pgf_ctx* ctx, int32_t slots, int32_t node_rows);
+PGF_API int pgf_get_gather_layout(pgf_ctx* ctx, int32_t* slots, int32_t* node_rows);
 
 #ifdef __cplusplus
 }

@@ -273,15 +273,18 @@ class DeviceContext:
         return tuple(out)
 
     @property
-    def gather_slots(self):
-        """Threads per matrix row of the row-gather assembly pass."""
-        v = C.c_int32()
-        self._chk(self.lib.pgf_get_gather_slots(self.h, C.byref(v)))
-        return v.value
+    def gather_layout(self):
+        """(slots, node_rows) of the row-gather assembly pass: threads sharing
+        one unit's incident elements, and whether a unit is a matrix row (0)
+        or all rows of a node (1)."""
+        sl, nr = C.c_int32(), C.c_int32()
+        self._chk(self.lib.pgf_get_gather_layout(self.h, C.byref(sl), C.byref(nr)))
+        return sl.value, nr.value
 
-    @gather_slots.setter
-    def gather_slots(self, slots):
-        self._chk(self.lib.pgf_set_gather_slots(self.h, int(slots)))
+    @gather_layout.setter
+    def gather_layout(self, layout):
+        slots, node_rows = layout
+        self._chk(self.lib.pgf_set_gather_layout(self.h, int(slots), int(node_rows)))
 
     def time_spmv(self, precision=64, reps=20):
         ms = C.c_double()

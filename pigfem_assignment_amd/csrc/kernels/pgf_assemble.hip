@@ -22,6 +22,12 @@
 #include "pgf_kernels.hpp"
 #include "pgf_device.hpp"
 
+// Multiply-adds are fused only within one expression (never across
+// statements), so the element arithmetic, and with it every assembled bit,
+// is the same in every kernel variant and gather layout that evaluates the
+// same expressions.
+#pragma clang fp contract(on)
+
 namespace pgfk {
 
 // An isotropic D (src/material_lei.cpp:52-97, every material of the path)
@@ -519,21 +525,25 @@ __global__ __launch_bounds__(cut_threads<DIM>()) void k_cut_elem(AsmArgs a) {
     }
 }
 
-// Gather scheme, pass 2.  A block owns a run of consecutive nodes; every
-// matrix row of those nodes is served by SL threads (slots, 1, 2, 4 or 8, a
-// per-context choice), slot s taking the row's incident elements s, s + SL, ...
-// so the element work of one row runs in parallel.  When the free rows of the block form one contiguous CSR segment
-// [s0, s1) it is accumulated in LDS and written back with coalesced stores;
-// otherwise (s0 == s1), and for prescribed rows (all_rows only), the rows are
-// accumulated in place in HBM.  The slots add their element's contribution in
-// slot order between barriers, so every entry sums its incident elements in
-// element order, the reference's summation order, and the result does not
-// depend on scheduling (with one slot a thread owns its row and needs no
-// barriers between elements).
-template <int DIM, int SL, bool DO_K, bool DO_F, typename KT>
+// Gather scheme, pass 2.  A block owns a run of consecutive nodes.  Its
+// threads are (unit, slot) pairs: a unit is one matrix row (NR = 1) or all DIM
+// rows of one node (NR = DIM, the element geometry and the nodal data then
+// serve the node's rows together), and the SL slots of a unit (1, 2, 4 or 8)
+// take the unit's incident elements s, s + SL, ... so the element work of one
+// node runs in parallel.  Rows, slots and rows per thread are a per-context
+// choice; every choice gives the same bits.  When the free rows of the block
+// form one contiguous CSR segment [s0, s1) it is accumulated in LDS and
+// written back with coalesced stores; otherwise (s0 == s1), and for
+// prescribed rows (all_rows only), the rows are accumulated in place in HBM.
+// The slots add their element's contribution in slot order between barriers,
+// so every entry sums its incident elements in element order, the reference's
+// summation order, and the result does not depend on scheduling (with one
+// slot a thread owns its rows and needs no barriers between elements).
+template <int DIM, int SL, int NR, bool DO_K, bool DO_F, typename KT>
 __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     HIP_DYNAMIC_SHARED(double, s_seg)
-    __shared__ double s_P[GATHER_THREADS / SL];
+    constexpr int NU = GATHER_THREADS / SL;  // units per block
+    __shared__ double s_P[NU * NR];
     const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
     if (blk >= a.n_gblk) return;  // padding block of the XCD mapping
     const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
@@ -541,37 +551,44 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     const int rounds = a.blk_rounds[blk];
     const bool use_lds = DO_K && s1 > s0;
     const int t = threadIdx.x;
-    const int lr = t / SL, slot = t % SL;  // local row, slot
-    const int64_t n = n0 + lr / DIM;
-    const int da = lr % DIM;
-    const bool active = lr < GATHER_THREADS / SL && n < n1;
+    const int lu = t / SL, slot = t % SL;  // local unit, slot
+    const int64_t n = NR == 1 ? n0 + lu / DIM : n0 + lu;
+    const bool active = n < n1;
     constexpr int S = DIM == 2 ? 3 : 6;
     constexpr int NQ = DIM == 2 ? 4 : 1;
-    const int r = active ? a.dof[da * a.n_all + n] : 0;
-    const bool own = active && (r < a.n_free || a.all_rows);
-    const bool in_lds = use_lds && r < a.n_free;
+    int da[NR], r[NR];
+    bool own[NR], in_lds[NR];
+    int64_t base[NR];
+    bool any = false;
+    for (int h = 0; h < NR; ++h) {
+        da[h] = NR == 1 ? lu % DIM : h;
+        r[h] = active ? a.dof[da[h] * a.n_all + n] : 0;
+        own[h] = active && (r[h] < a.n_free || a.all_rows);
+        in_lds[h] = use_lds && r[h] < a.n_free;
+        base[h] = own[h] ? a.rowptr[r[h]] : 0;
+        any = any || own[h];
+    }
     if (use_lds)
         for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) s_seg[j] = 0.0;
-    if (DO_K && own && !in_lds && slot == 0)
-        for (int32_t j = a.rowptr[r]; j < a.rowptr[r + 1]; ++j) {
-            if (sizeof(KT) == 8) a.K[j] = 0.0;
-            else a.K32[j] = 0.0f;
-        }
-    if (DO_F && slot == 0 && lr < GATHER_THREADS / SL) s_P[lr] = 0.0;
+    if (DO_K && slot == 0)
+        for (int h = 0; h < NR; ++h)
+            if (own[h] && !in_lds[h])
+                for (int32_t j = a.rowptr[r[h]]; j < a.rowptr[r[h] + 1]; ++j) {
+                    if (sizeof(KT) == 8) a.K[j] = 0.0;
+                    else a.K32[j] = 0.0f;
+                }
+    if (DO_F && slot == 0)
+        for (int h = 0; h < NR; ++h) s_P[lu * NR + h] = 0.0;
     __syncthreads();
-    const int64_t kbeg = own ? a.inc_ptr[n] : 0, kend = own ? a.inc_ptr[n + 1] : 0;
-    const int64_t base = own ? a.rowptr[r] : 0;
-    auto add = [&](int64_t pos, double v) {
-        if (in_lds) s_seg[pos - s0] += v;
-        else kadd<KT>(a, pos, v);
-    };
+    const int64_t kbeg = any ? a.inc_ptr[n] : 0, kend = any ? a.inc_ptr[n + 1] : 0;
     for (int rr = 0; rr < rounds; ++rr) {
         const int64_t k = kbeg + (int64_t)rr * SL + slot;
-        const bool valid = own && k < kend;
-        // ---- compute: this slot's element contribution to the row ----
-        double Kr[4 * DIM];
+        const bool valid = any && k < kend;
+        // ---- compute: this slot's element contribution to the unit's rows ----
+        double Kr[NR][4 * DIM];
         int nd[4];
-        double Pc = 0.0;
+        double Pc[NR];
+        for (int h = 0; h < NR; ++h) Pc[h] = 0.0;
         int c = -1, la = 0;
         if (valid) {
             const int e = a.inc_elem[k];
@@ -589,7 +606,8 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                 if (DO_F)
                     for (int q = 0; q < 4; ++q)
                         for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
-                for (int j = 0; j < 4 * DIM; ++j) Kr[j] = 0.0;
+                for (int h = 0; h < NR; ++h)
+                    for (int j = 0; j < 4 * DIM; ++j) Kr[h][j] = 0.0;
                 // one quadrature point at a time keeps the register footprint small
 #pragma unroll 1
                 for (int q = 0; q < NQ; ++q) {
@@ -600,28 +618,37 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                     double gr[DIM];
                     for (int i = 0; i < DIM; ++i)
                         gr[i] = la == 0 ? g[i] : la == 1 ? g[DIM + i] : la == 2 ? g[2 * DIM + i] : g[3 * DIM + i];
+                    double sig[S];
                     if (DO_F) {
-                        double eps[S], sig[S];
+                        double eps[S];
                         strain_of<DIM>(4, g, u, eps);
                         stress_of<DIM>(D, eps, sig);
-                        Pc += w * force_entry<DIM>(gr, da, sig);
                     }
-                    if (DO_K) {
-                        double parts[S];
-                        k_parts<DIM>(gr, da, D, parts);
-                        for (int bn = 0; bn < 4; ++bn)
-                            for (int db = 0; db < DIM; ++db) Kr[bn * DIM + db] += w * k_entry<DIM>(parts, &g[bn * DIM], db);
+                    for (int h = 0; h < NR; ++h) {
+                        if (DO_F) Pc[h] += w * force_entry<DIM>(gr, da[h], sig);
+                        if (DO_K) {
+                            double parts[S];
+                            k_parts<DIM>(gr, da[h], D, parts);
+                            for (int bn = 0; bn < 4; ++bn)
+                                for (int db = 0; db < DIM; ++db)
+                                    Kr[h][bn * DIM + db] += w * k_entry<DIM>(parts, &g[bn * DIM], db);
+                        }
                     }
                 }
             } else if (DO_F) {
-                Pc = a.pe[a.pe_ptr[c] + la * DIM + da];
+                for (int h = 0; h < NR; ++h) Pc[h] = a.pe[a.pe_ptr[c] + la * DIM + da[h]];
             }
         }
         // ---- add, slot by slot (element order) ----
         for (int ss = 0; ss < SL; ++ss) {
             if (valid && slot == ss) {
-                if (DO_F) s_P[lr] += Pc;
+                if (DO_F)
+                    for (int h = 0; h < NR; ++h) s_P[lu * NR + h] += Pc[h];
                 if (DO_K) {
+                    auto add = [&](int h, int64_t rel, double v) {
+                        if (in_lds[h]) s_seg[base[h] + rel - s0] += v;
+                        else kadd<KT>(a, base[h] + rel, v);
+                    };
                     if (c < 0) {
                         // column positions from the packed per-node offsets
                         // (free and prescribed part of the row)
@@ -631,12 +658,13 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                             int cf = 0, cp = 0;
                             for (int db = 0; db < DIM; ++db) {
                                 const int cdof = a.dof[db * a.n_all + nd[bn]];
-                                add(cdof < a.n_free ? base + of + cf++ : base + op + cp++, Kr[bn * DIM + db]);
+                                const int rel = cdof < a.n_free ? of + cf++ : op + cp++;
+                                for (int h = 0; h < NR; ++h)
+                                    if (own[h]) add(h, rel, Kr[h][bn * DIM + db]);
                             }
                         }
                     } else {
                         const int nt = a.ce_nt[c], M = nt * DIM;
-                        const double* krow = a.ke + a.ke_ptr[c] + (int64_t)(la * DIM + da) * M;
                         const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
                         for (int bn = 0; bn < nt; ++bn) {
                             const int nb = a.ce_nodes[c * 16 + bn];
@@ -644,8 +672,9 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                             int cf = 0, cp = 0;
                             for (int db = 0; db < DIM; ++db) {
                                 const int cdof = a.dof[db * a.n_all + nb];
-                                const int64_t pos = cdof < a.n_free ? base + of + cf++ : base + op + cp++;
-                                add(pos, krow[bn * DIM + db]);
+                                const int rel = cdof < a.n_free ? of + cf++ : op + cp++;
+                                for (int h = 0; h < NR; ++h)
+                                    if (own[h]) add(h, rel, a.ke[a.ke_ptr[c] + (int64_t)(la * DIM + da[h]) * M + bn * DIM + db]);
                             }
                         }
                     }
@@ -654,7 +683,9 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
             if (SL > 1) __syncthreads();
         }
     }
-    if (DO_F && own && slot == 0) a.P[r] = s_P[lr];
+    if (DO_F && slot == 0)
+        for (int h = 0; h < NR; ++h)
+            if (own[h]) a.P[r[h]] = s_P[lu * NR + h];
     if (use_lds) {
         __syncthreads();
         for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) {
@@ -694,12 +725,23 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
     const dim3 grid((unsigned)pgfk::xcd_blocks(c.n_gblk)), blk(pgfk::GATHER_THREADS);
-    switch (c.gslots) {
-        case 1: hipLaunchKernelGGL((pgfk::k_gather<DIM, 1, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
-        case 2: hipLaunchKernelGGL((pgfk::k_gather<DIM, 2, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
-        case 4: hipLaunchKernelGGL((pgfk::k_gather<DIM, 4, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
-        default: hipLaunchKernelGGL((pgfk::k_gather<DIM, 8, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a); break;
+#define PGF_GATHER(SL, NR) hipLaunchKernelGGL((pgfk::k_gather<DIM, SL, NR, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a)
+    if (c.gnode) {
+        switch (c.gslots) {
+            case 1: PGF_GATHER(1, DIM); break;
+            case 2: PGF_GATHER(2, DIM); break;
+            case 4: PGF_GATHER(4, DIM); break;
+            default: PGF_GATHER(8, DIM); break;
+        }
+    } else {
+        switch (c.gslots) {
+            case 1: PGF_GATHER(1, 1); break;
+            case 2: PGF_GATHER(2, 1); break;
+            case 4: PGF_GATHER(4, 1); break;
+            default: PGF_GATHER(8, 1); break;
+        }
     }
+#undef PGF_GATHER
     if (c.timing_events) (void)hipEventRecord(c.timing_events[2], c.stream);
 }
 

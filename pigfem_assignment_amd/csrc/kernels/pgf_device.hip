@@ -145,17 +145,19 @@ __device__ inline void sum_partials(int nb, const double* partial, double* v) {
     block_reduce<NV>(v);
 }
 
-// Gather blocks: runs of consecutive nodes, `slots` threads per matrix row,
-// GATHER_THREADS / (slots * dim) nodes per block.  A block's free rows are
+// Gather blocks: runs of consecutive nodes, `slots` threads per matrix row
+// (node_rows = 0) or per node (node_rows = 1, one thread serves all rows of a
+// node), GATHER_THREADS / (slots * dim) or GATHER_THREADS / slots nodes per
+// block.  A block's free rows are
 // staged in LDS when they form one contiguous CSR segment of at most SEG_MAX
 // entries; blk_rounds holds the incident elements of the block's busiest node
 // divided by the slots (rounded up).
-int default_gather_slots(int dim) { return dim == 2 ? 1 : 4; }
+int default_gather_slots(int dim) { return dim == 2 ? 1 : 2; }
 
-void build_gather_blocks(Ctx& c, int slots) {
+void build_gather_blocks(Ctx& c, int slots, bool node_rows) {
     if (slots != 1 && slots != 2 && slots != 4 && slots != 8)
         throw pgf::Error(PGF_EARG, "gather slots must be 1, 2, 4 or 8");
-    const int per_blk = pgfk::GATHER_THREADS / (slots * c.dim);
+    const int per_blk = pgfk::GATHER_THREADS / (slots * (node_rows ? 1 : c.dim));
     const int64_t na = c.n_all;
     const std::vector<int64_t>& rp = c.h_rowptr;
     std::vector<int64_t> bn{0}, bs;
@@ -197,6 +199,7 @@ void build_gather_blocks(Ctx& c, int slots) {
     c.n_gblk = (int64_t)bn.size() - 1;
     c.seg_max = seg_max;
     c.gslots = slots;
+    c.gnode = node_rows;
 }
 
 // scalar slots of the PCG recurrence
@@ -753,7 +756,7 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->pe = pgfd::dalloc<double>(pp[d->n_cut]);
             c->h_dof.assign(d->dof, d->dof + (size_t)d->dim * d->n_all_nodes);
             c->h_inc_ptr = pat.inc_ptr;
-            pgfd::build_gather_blocks(*c, pgfd::default_gather_slots(d->dim));
+            pgfd::build_gather_blocks(*c, pgfd::default_gather_slots(d->dim), false);
         }
         c->dmax_conv = pgfd::dalloc<double>(n_cpt);
         c->dmax_trial = pgfd::dalloc<double>(n_cpt);
@@ -1175,18 +1178,19 @@ PGF_API int pgf_time_assemble_split(pgf_ctx* c, uint32_t flags, int32_t reps, do
     });
 }
 
-PGF_API int pgf_set_gather_slots(pgf_ctx* c, int32_t slots) {
-    if (!c) return PGF_EARG;
+PGF_API int pgf_set_gather_layout(pgf_ctx* c, int32_t slots, int32_t node_rows) {
+    if (!c || (node_rows != 0 && node_rows != 1)) return PGF_EARG;
     return guard(c, [&]() -> int {
         HIPCHK(hipStreamSynchronize(c->stream));
-        pgfd::build_gather_blocks(*c, slots);
+        pgfd::build_gather_blocks(*c, slots, node_rows != 0);
         return 0;
     });
 }
 
-PGF_API int pgf_get_gather_slots(pgf_ctx* c, int32_t* slots) {
-    if (!c || !slots) return PGF_EARG;
+PGF_API int pgf_get_gather_layout(pgf_ctx* c, int32_t* slots, int32_t* node_rows) {
+    if (!c || !slots || !node_rows) return PGF_EARG;
     *slots = c->gslots;
+    *node_rows = c->gnode ? 1 : 0;
     return 0;
 }
 

@@ -51,7 +51,8 @@ struct Ctx {
     int64_t *blk_node = nullptr, *blk_seg = nullptr;
     int32_t* blk_rounds = nullptr;
     int64_t n_gblk = 0, seg_max = 0;
-    int gslots = 1;                 // threads per matrix row in the gather pass
+    int gslots = 1;                 // gather pass: threads per row (or per node with gnode)
+    bool gnode = false;             // gather pass: one thread serves all rows of a node
     std::vector<int32_t> h_dof;      // [dim][n_all] (gather partition)
     std::vector<int64_t> h_inc_ptr;  // [n_all+1] node -> element incidence
     // free / prescribed dof -> nodal field slot (d*n_all + node)
@@ -85,7 +86,7 @@ template <typename T>
 void fill_async(Ctx& c, T* p, int64_t n, T v);
 void enqueue_assembly(Ctx& c, double dt, uint32_t flags);
 int default_gather_slots(int dim);
-void build_gather_blocks(Ctx& c, int slots);
+void build_gather_blocks(Ctx& c, int slots, bool node_rows);
 
 // Learned initial guess (pgf_cnn.hip).  Called after the Jacobian assembly of
 // Newton iteration `it` with that iteration's residual in c.r.  Iteration 0

@@ -156,10 +156,11 @@ def test_3d_cohesive_stiffness(law, scheme):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-def test_gather_slots_bit_identical(dim):
-    """Every slot count of the gather pass sums each entry's element
-    contributions in element order: K, P and the cohesive state are bit-identical
-    across 1, 2, 4 and 8 threads per row, and match the oracle."""
+def test_gather_layouts_bit_identical(dim):
+    """Every layout of the gather pass (1, 2, 4, 8 threads per unit; a unit one
+    matrix row or all rows of a node) sums each entry's element contributions
+    in element order: K, P and the cohesive state are bit-identical across
+    layouts, and match the oracle."""
     p, o = _c0(60) if dim == 2 else _c2small(configs.BILINEAR_C2)
     ctx = DeviceContext(p)
     rng = np.random.RandomState(5)
@@ -169,23 +170,26 @@ def test_gather_slots_bit_identical(dim):
     vals, Po, _ = o.assemble(dt=0.05)
     rp = ctx.pattern()[0]
     ref = None
-    for slots in (1, 2, 4, 8):
-        ctx.gather_slots = slots
-        assert ctx.gather_slots == slots
-        for flags in (L.PGF_ASM_JAC | L.PGF_ASM_FUNC | L.PGF_ASM_ALL_ROWS, L.PGF_ASM_JAC, L.PGF_ASM_FUNC):
-            ctx.assemble(flags, dt=0.05)
-        ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FUNC | L.PGF_ASM_ALL_ROWS, dt=0.05)
-        got = (ctx.matrix(), ctx.internal_force(), ctx.isv()[0])
-        print("dim %d slots %d" % (dim, slots), flush=True)
-        if ref is None:
-            ref = got
-            assert entrywise_ok(got[0], vals, rp)[0] <= 1.0
-            assert np.max(np.abs(got[1] - Po)) <= 1e-12 * np.max(np.abs(Po))
-        else:
-            for g, r in zip(got, ref):
-                assert np.array_equal(g, r)
+    for node_rows in (0, 1):
+        for slots in (1, 2, 4, 8):
+            ctx.gather_layout = (slots, node_rows)
+            assert ctx.gather_layout == (slots, node_rows)
+            for flags in (L.PGF_ASM_JAC | L.PGF_ASM_FUNC | L.PGF_ASM_ALL_ROWS, L.PGF_ASM_JAC, L.PGF_ASM_FUNC):
+                ctx.assemble(flags, dt=0.05)
+            ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FUNC | L.PGF_ASM_ALL_ROWS, dt=0.05)
+            got = (ctx.matrix(), ctx.internal_force(), ctx.isv()[0])
+            print("dim %d slots %d node_rows %d" % (dim, slots, node_rows), flush=True)
+            if ref is None:
+                ref = got
+                assert entrywise_ok(got[0], vals, rp)[0] <= 1.0
+                assert np.max(np.abs(got[1] - Po)) <= 1e-12 * np.max(np.abs(Po))
+            else:
+                for g, r in zip(got, ref):
+                    assert np.array_equal(g, r)
     with pytest.raises(RuntimeError):
-        ctx.gather_slots = 3
+        ctx.gather_layout = (3, 0)
+    with pytest.raises(RuntimeError):
+        ctx.gather_layout = (1, 2)
 
 
 @pytest.mark.parametrize("precision", [64, 32])
