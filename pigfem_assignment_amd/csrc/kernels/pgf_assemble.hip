@@ -650,18 +650,14 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                         else kadd<KT>(a, base[h] + rel, v);
                     };
                     if (c < 0) {
-                        // column positions from the packed per-node offsets
-                        // (free and prescribed part of the row)
-                        const uint64_t off = a.inc_off[k];
-                        for (int bn = 0; bn < 4; ++bn) {
-                            const int of = (int)((off >> (16 * bn)) & 0xff), op = (int)((off >> (16 * bn + 8)) & 0xff);
-                            int cf = 0, cp = 0;
-                            for (int db = 0; db < DIM; ++db) {
-                                const int cdof = a.dof[db * a.n_all + nd[bn]];
-                                const int rel = cdof < a.n_free ? of + cf++ : op + cp++;
-                                for (int h = 0; h < NR; ++h)
-                                    if (own[h]) add(h, rel, Kr[h][bn * DIM + db]);
-                            }
+                        // row-relative column positions, one byte per column
+                        uint32_t wv[DIM];
+                        const uint32_t* rw = (const uint32_t*)(a.inc_rel + k * (4 * DIM));
+                        for (int i = 0; i < DIM; ++i) wv[i] = rw[i];
+                        for (int j = 0; j < 4 * DIM; ++j) {
+                            const int rel = (int)((wv[j / 4] >> (8 * (j % 4))) & 0xffu);
+                            for (int h = 0; h < NR; ++h)
+                                if (own[h]) add(h, rel, Kr[h][j]);
                         }
                     } else {
                         const int nt = a.ce_nt[c], M = nt * DIM;

@@ -61,7 +61,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.dmax_conv = dmax_conv; a.dmax_trial = dmax_trial; a.flags = flags;
     a.rowptr = rowptr; a.K = K; a.K32 = K32; a.P = P;
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index;
-    a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_off = inc_off;
+    a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_rel = inc_rel;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
     return a;
@@ -73,7 +73,7 @@ Ctx::~Ctx() {
                     dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
-                    inc_off, ke, pe, blk_node, blk_seg, blk_rounds, grp_row};
+                    inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -742,7 +742,33 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->inc_ptr = pgfd::dupload(pat.inc_ptr.data(), (int64_t)pat.inc_ptr.size());
             c->inc_elem = pgfd::dupload(pat.inc_elem.data(), c->n_inc);
             c->inc_slot = pgfd::dupload(pat.inc_slot.data(), c->n_inc);
-            c->inc_off = pgfd::dupload(pat.inc_off.data(), c->n_inc);
+            // per incidence of an uncut element: the row-relative position of
+            // each of the element's 4 * dim columns (free part, then
+            // prescribed part of the row), one byte each, so the gather pass
+            // needs no dof lookups to place its entries
+            {
+                const int W = 4 * d->dim;
+                std::vector<uint8_t> rel((size_t)std::max<int64_t>(c->n_inc, 1) * W, 0);
+                bool wide = false;
+                for (int64_t k = 0; k < c->n_inc; ++k) {
+                    const int64_t e = pat.inc_elem[k];
+                    if (mv.cut_index[e] >= 0) continue;
+                    const uint64_t off = pat.inc_off[k];
+                    for (int bn = 0; bn < 4; ++bn) {
+                        const int of = (int)((off >> (16 * bn)) & 0xff), op = (int)((off >> (16 * bn + 8)) & 0xff);
+                        const int64_t nd = d->conn[e * 4 + bn];
+                        int cf = 0, cp = 0;
+                        for (int db = 0; db < d->dim; ++db) {
+                            const int64_t g = d->dof[(size_t)db * d->n_all_nodes + nd];
+                            const int v = g < d->n_free ? of + cf++ : op + cp++;
+                            if (v > 255) wide = true;
+                            rel[(size_t)k * W + bn * d->dim + db] = (uint8_t)v;
+                        }
+                    }
+                }
+                if (wide) throw pgf::Error(PGF_ELIMIT, "a matrix row is wider than 256 columns");
+                c->inc_rel = pgfd::dupload(rel.data(), (int64_t)rel.size());
+            }
             std::vector<int64_t> kp(d->n_cut + 1, 0), pp(d->n_cut + 1, 0);
             for (int64_t k = 0; k < d->n_cut; ++k) {
                 const int64_t m = (int64_t)ce_nt[k] * d->dim;

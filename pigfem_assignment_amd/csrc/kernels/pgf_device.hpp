@@ -45,7 +45,7 @@ struct Ctx {
     int32_t *el_conn = nullptr, *cut_index = nullptr, *inc_elem = nullptr;
     uint8_t *el_mat = nullptr, *inc_slot = nullptr;
     int64_t *inc_ptr = nullptr, *ke_ptr = nullptr, *pe_ptr = nullptr;
-    uint64_t* inc_off = nullptr;
+    uint8_t* inc_rel = nullptr;  // [n_inc][4 * dim] row-relative column positions (uncut elements)
     double *ke = nullptr, *pe = nullptr;
     int64_t n_inc = 0, n_ke = 0;
     int64_t *blk_node = nullptr, *blk_seg = nullptr;

@@ -88,7 +88,7 @@ struct AsmArgs {
     const int64_t* inc_ptr;    // [n_all+1]
     const int32_t* inc_elem;
     const uint8_t* inc_slot;
-    const uint64_t* inc_off;
+    const uint8_t* inc_rel;    // [n_inc][4 * dim] row-relative column positions (uncut elements)
     double* ke;                // cut-element K_e scratch
     const int64_t* ke_ptr;
     double* pe;                // cut-element P_e scratch
