@@ -661,17 +661,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                         }
                     } else {
                         const int nt = a.ce_nt[c], M = nt * DIM;
-                        const uint8_t* off = a.ce_off + a.ce_off_ptr[c];
-                        for (int bn = 0; bn < nt; ++bn) {
-                            const int nb = a.ce_nodes[c * 16 + bn];
-                            const int of = off[(la * nt + bn) * 2], op = off[(la * nt + bn) * 2 + 1];
-                            int cf = 0, cp = 0;
-                            for (int db = 0; db < DIM; ++db) {
-                                const int cdof = a.dof[db * a.n_all + nb];
-                                const int rel = cdof < a.n_free ? of + cf++ : op + cp++;
-                                for (int h = 0; h < NR; ++h)
-                                    if (own[h]) add(h, rel, a.ke[a.ke_ptr[c] + (int64_t)(la * DIM + da[h]) * M + bn * DIM + db]);
-                            }
+                        const uint8_t* rl = a.ce_rel + a.ce_rel_ptr[c] + (int64_t)la * M;
+                        const double* kb = a.ke + a.ke_ptr[c];
+                        for (int j = 0; j < M; ++j) {
+                            const int rel = rl[j];
+                            for (int h = 0; h < NR; ++h)
+                                if (own[h]) add(h, rel, kb[(int64_t)(la * DIM + da[h]) * M + j]);
                         }
                     }
                 }

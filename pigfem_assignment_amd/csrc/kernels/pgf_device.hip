@@ -57,7 +57,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.ce_list = ce_list; a.ce_nodes = ce_nodes; a.ce_nt = ce_nt;
     a.vpt_ptr = vpt_ptr; a.vpt_w = vpt_w; a.vpt_mat = vpt_mat; a.vpt_grad_off = vpt_grad_off; a.vpt_grad = vpt_grad;
     a.cpt_ptr = cpt_ptr; a.cpt_w = cpt_w; a.cpt_N = cpt_N; a.cpt_R = cpt_R; a.cpt_nodes = cpt_nodes; a.cpt_nh = cpt_nh;
-    a.cpt_law = cpt_law; a.laws = laws; a.ce_off = ce_off; a.ce_off_ptr = ce_off_ptr;
+    a.cpt_law = cpt_law; a.laws = laws; a.ce_off = ce_off; a.ce_off_ptr = ce_off_ptr; a.ce_rel = ce_rel; a.ce_rel_ptr = ce_rel_ptr;
     a.dmax_conv = dmax_conv; a.dmax_trial = dmax_trial; a.flags = flags;
     a.rowptr = rowptr; a.K = K; a.K32 = K32; a.P = P;
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index;
@@ -73,7 +73,7 @@ Ctx::~Ctx() {
                     dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
-                    inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row};
+                    inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row, ce_rel, ce_rel_ptr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -726,6 +726,31 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         c->cpt_law = pgfd::dupload(d->cpt_law, n_cpt);
         c->ce_off = pgfd::dupload(pat.cut_off.data(), (int64_t)pat.cut_off.size());
         c->ce_off_ptr = pgfd::dupload(pat.cut_off_ptr.data(), (int64_t)pat.cut_off_ptr.size());
+        // cut elements: row-relative position of every column (bn, db) in the
+        // rows of slot la, one byte each, [nt][nt * dim] per element
+        {
+            std::vector<int64_t> rp(d->n_cut + 1, 0);
+            for (int64_t k = 0; k < d->n_cut; ++k) rp[k + 1] = rp[k] + (int64_t)ce_nt[k] * ce_nt[k] * d->dim;
+            std::vector<uint8_t> rel(std::max<int64_t>(rp[d->n_cut], 1), 0);
+            for (int64_t k = 0; k < d->n_cut; ++k) {
+                const int nt = ce_nt[k];
+                const uint8_t* off = pat.cut_off.data() + pat.cut_off_ptr[k];
+                for (int la = 0; la < nt; ++la)
+                    for (int bn = 0; bn < nt; ++bn) {
+                        const int of = off[(la * nt + bn) * 2], op = off[(la * nt + bn) * 2 + 1];
+                        const int64_t nd = ce_nodes[k * 16 + bn];
+                        int cf = 0, cp = 0;
+                        for (int db = 0; db < d->dim; ++db) {
+                            const int64_t g = d->dof[(size_t)db * d->n_all_nodes + nd];
+                            const int v = g < d->n_free ? of + cf++ : op + cp++;
+                            if (v > 255) throw pgf::Error(PGF_ELIMIT, "a matrix row is wider than 256 columns");
+                            rel[rp[k] + (int64_t)la * nt * d->dim + bn * d->dim + db] = (uint8_t)v;
+                        }
+                    }
+            }
+            c->ce_rel = pgfd::dupload(rel.data(), (int64_t)rel.size());
+            c->ce_rel_ptr = pgfd::dupload(rp.data(), d->n_cut + 1);
+        }
         // gather scheme: element-order connectivity / material, incidence plan,
         // dense scratch for the cut elements' K_e and P_e
         {

@@ -46,6 +46,8 @@ struct Ctx {
     uint8_t *el_mat = nullptr, *inc_slot = nullptr;
     int64_t *inc_ptr = nullptr, *ke_ptr = nullptr, *pe_ptr = nullptr;
     uint8_t* inc_rel = nullptr;  // [n_inc][4 * dim] row-relative column positions (uncut elements)
+    uint8_t* ce_rel = nullptr;   // per cut element [nt][nt * dim] row-relative column positions
+    int64_t* ce_rel_ptr = nullptr;
     double *ke = nullptr, *pe = nullptr;
     int64_t n_inc = 0, n_ke = 0;
     int64_t *blk_node = nullptr, *blk_seg = nullptr;

@@ -76,6 +76,8 @@ struct AsmArgs {
     const CohParams* laws;
     const uint8_t* ce_off;
     const int64_t* ce_off_ptr;
+    const uint8_t* ce_rel;      // per cut element [nt][nt * dim] row-relative column positions
+    const int64_t* ce_rel_ptr;
     const double* dmax_conv;
     double* dmax_trial;
     uint8_t* flags;
