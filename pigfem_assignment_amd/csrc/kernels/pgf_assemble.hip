@@ -333,18 +333,21 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_P,
     if (tid < nc) {
         const int64_t q = c0 + tid;
         const int nh = a.cpt_nh[q];
-        double N[4], R[9], dl[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
-        int map[8];
-        for (int k = 0; k < 4; ++k) N[k] = a.cpt_N[q * 4 + k];
+        double R[9], dl[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
         for (int k = 0; k < 9; ++k) R[k] = a.cpt_R[q * 9 + k];
-        for (int k = 0; k < 8; ++k) map[k] = k < 2 * nh ? (int)a.cpt_nodes[q * 8 + k] : -1;
-        for (int d = 0; d < DIM; ++d)
-            for (int n = 0; n < nh; ++n) dl[d] += N[n] * (s_u[map[nh + n] * DIM + d] - s_u[map[n] * DIM + d]);
+        // jump across the interface: surface node n (lower slot map[n], upper
+        // slot map[nh + n]) read straight from the tables, no indexed arrays
+        for (int n = 0; n < 4; ++n) {
+            if (n >= nh) break;
+            const double Nn = a.cpt_N[q * 4 + n];
+            const int lo = a.cpt_nodes[q * 8 + n], hi = a.cpt_nodes[q * 8 + nh + n];
+            for (int d = 0; d < DIM; ++d) dl[d] += Nn * (s_u[hi * DIM + d] - s_u[lo * DIM + d]);
+        }
         for (int i = 0; i < DIM; ++i)
             for (int j = 0; j < DIM; ++j) delta[i] += R[i * DIM + j] * dl[j];
         double trac[3], Dc[9], dnew;
         uint8_t fl;
-        coh_law(a.laws[a.cpt_law[q]], DIM, delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
+        coh_law<DIM>(a.laws[a.cpt_law[q]], delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
         a.dmax_trial[q] = dnew;
         a.flags[q] = fl;
         double RtD[9];
@@ -365,8 +368,8 @@ __device__ inline void cut_compute(const AsmArgs& a, int c, int nt, double* s_P,
             for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * trac[k];
             s_ctR[tid * DIM + i] = s;
         }
-        for (int k = 0; k < 4; ++k) s_cN[tid * 4 + k] = N[k];
-        for (int k = 0; k < 8; ++k) s_cmap[tid * 8 + k] = map[k];
+        for (int k = 0; k < 4; ++k) s_cN[tid * 4 + k] = a.cpt_N[q * 4 + k];
+        for (int k = 0; k < 8; ++k) s_cmap[tid * 8 + k] = k < 2 * nh ? (int)a.cpt_nodes[q * 8 + k] : -1;
         s_cnh[tid] = nh;
         s_cw[tid] = a.cpt_w[q];
     }

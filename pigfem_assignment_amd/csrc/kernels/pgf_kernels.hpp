@@ -113,8 +113,10 @@ struct AsmArgs {
 // :115), delta_eff from src/Material_OPCohesiveNU.cpp:45-63, damage zone from
 // :224-236.  Bilinear: the commented law of :175-199/256-271 with delta_c and
 // delta_f as parameters.  The irreversibility test and the delta_max update are
-// FP64 in every precision mode.
-__device__ inline void coh_law(const CohParams& L, int dim, double* delta, double dmax, double dt,
+// FP64 in every precision mode.  The dimension is a template parameter so
+// every small array stays in registers.
+template <int dim>
+__device__ inline void coh_law(const CohParams& L, double* delta, double dmax, double dt,
                                double* trac, double* Dc, double& dmax_new, uint8_t& flag) {
     const double b2 = L.beta * L.beta;
     double norm = 0.0;
