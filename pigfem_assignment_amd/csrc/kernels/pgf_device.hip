@@ -134,13 +134,23 @@ __device__ inline void block_partial(double* v, double* partial) {
         for (int k = 0; k < NV; ++k) partial[k * gridDim.x + blockIdx.x] = v[k];
 }
 
-// Fixed-order sum of nb partial sums of NV values by one block.
+// Fixed-order sum of nb partial sums of NV values by one block; four
+// independent accumulators per thread keep four loads in flight.
 template <int NV>
 __device__ inline void sum_partials(int nb, const double* partial, double* v) {
+    const int bd = blockDim.x;
     for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[k * nb + i];
-        v[k] = s;
+        const double* pk = partial + (int64_t)k * nb;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int i = threadIdx.x;
+        for (; i + 3 * bd < nb; i += 4 * bd) {
+            s0 += pk[i];
+            s1 += pk[i + bd];
+            s2 += pk[i + 2 * bd];
+            s3 += pk[i + 3 * bd];
+        }
+        for (; i < nb; i += bd) s0 += pk[i];
+        v[k] = (s0 + s1) + (s2 + s3);
     }
     block_reduce<NV>(v);
 }
