@@ -46,7 +46,7 @@ struct BArgs {
     const int64_t* sys_tile;   // [n_sys + 1] tile range of each system
     double* scal;              // [n_sys][NR][8]: rz, pq, alpha, beta, rr, rr0, tol2, iters
     int* done;                 // [n_sys][NR]
-    int* active;               // [1] number of pairs still running
+    int* active;               // [1] nonzero while some pair is still running
     int n_sys;
 };
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_RR0, S_TOL2, S_IT, S_N };
@@ -120,18 +120,23 @@ __global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
         for (int c = 0; c < NR; ++c) a.tpart[t * NR + c] = tot[c];
 }
 
-// per system: pq = sum of its tiles' partials (tile order), alpha = rz / pq
+// per (system, right-hand side) pair, one wavefront: pq = sum of the
+// system's tile partials (lane-strided, then a fixed butterfly), alpha = rz / pq
 template <int NR>
-__global__ void k_breduce_pq(BArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.n_sys * NR) return;
+__global__ __launch_bounds__(256) void k_breduce_pq(BArgs a) {
+    const int k = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    const int lane = threadIdx.x & 63;
+    if (k >= a.n_sys * NR) return;  // uniform per wavefront
     const int s = k / NR, c = k % NR;
     if (a.done[k]) return;
     double pq = 0.0;
-    for (int64_t t = a.sys_tile[s]; t < a.sys_tile[s + 1]; ++t) pq += a.tpart[t * NR + c];
-    double* sc = a.scal + (size_t)k * S_N;
-    sc[S_PQ] = pq;
-    sc[S_ALPHA] = pq != 0.0 ? sc[S_RZ] / pq : 0.0;
+    for (int64_t t = a.sys_tile[s] + lane; t < a.sys_tile[s + 1]; t += 64) pq += a.tpart[t * NR + c];
+    for (int m = 32; m >= 1; m >>= 1) pq += __shfl_xor(pq, m, 64);
+    if (lane == 0) {
+        double* sc = a.scal + (size_t)k * S_N;
+        sc[S_PQ] = pq;
+        sc[S_ALPHA] = pq != 0.0 ? sc[S_RZ] / pq : 0.0;
+    }
 }
 
 // x += alpha p, r -= alpha q, z = dinv r; partial r.z and r.r
@@ -172,19 +177,25 @@ __global__ __launch_bounds__(BT) void k_bupdate(BArgs a) {
         }
 }
 
-// per system: beta = rz_new / rz, convergence test rr <= tol2, iteration
-// count; every pair still running afterwards adds 1 to *active
+// per pair, one wavefront: beta = rz_new / rz, convergence test rr <= tol2,
+// iteration count; every pair still running afterwards sets *active
 template <int NR>
-__global__ void k_breduce_rz(BArgs a, int maxit) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.n_sys * NR) return;
+__global__ __launch_bounds__(256) void k_breduce_rz(BArgs a, int maxit) {
+    const int k = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    const int lane = threadIdx.x & 63;
+    if (k >= a.n_sys * NR) return;  // uniform per wavefront
     if (a.done[k]) return;
     const int s = k / NR, c = k % NR;
     double rz = 0.0, rr = 0.0;
-    for (int64_t t = a.sys_tile[s]; t < a.sys_tile[s + 1]; ++t) {
+    for (int64_t t = a.sys_tile[s] + lane; t < a.sys_tile[s + 1]; t += 64) {
         rz += a.tpart2[t * 2 * NR + c];
         rr += a.tpart2[t * 2 * NR + NR + c];
     }
+    for (int m = 32; m >= 1; m >>= 1) {
+        rz += __shfl_xor(rz, m, 64);
+        rr += __shfl_xor(rr, m, 64);
+    }
+    if (lane != 0) return;
     double* sc = a.scal + (size_t)k * S_N;
     sc[S_BETA] = sc[S_RZ] != 0.0 ? rz / sc[S_RZ] : 0.0;
     sc[S_RZ] = rz;
@@ -193,7 +204,7 @@ __global__ void k_breduce_rz(BArgs a, int maxit) {
     if (rr <= sc[S_TOL2] || sc[S_IT] >= maxit || !(rr == rr)) {
         a.done[k] = 1;
     } else {
-        atomicAdd(a.active, 1);
+        *a.active = 1;
     }
 }
 
@@ -344,7 +355,7 @@ struct BatchRun {
 template <int NR, int G>
 void enqueue_iteration(BArgs& a, hipStream_t st, int maxit) {
     const unsigned nt = (unsigned)a.n_tiles;
-    const unsigned np = (unsigned)((a.n_sys * NR + 255) / 256);
+    const unsigned np = (unsigned)(((int64_t)a.n_sys * NR * 64 + 255) / 256);  // one wavefront per pair
     hipLaunchKernelGGL((k_bspmm<NR, G>), dim3(nt), dim3(BT), 0, st, a);
     hipLaunchKernelGGL(k_breduce_pq<NR>, dim3(np), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_bupdate<NR>, dim3(nt), dim3(BT), 0, st, a);
