@@ -9,7 +9,9 @@
 //    together, one SpMM sweep per iteration reads each K entry once for all NR
 //    vectors; row tiles never cross a system boundary, tiles of converged
 //    systems are skipped, so the work is the sum over systems of their own
-//    iteration counts;
+//    iteration counts; when the live tiles fall below 60 % of those launched
+//    the launch list is compacted to the live ones (and the graph captured
+//    again), so converged systems stop costing workgroup launches;
 //  * a chunk of iterations is captured once in a hipGraph and replayed until
 //    every pair has converged (or hit maxit); clearing is done by kernels, the
 //    graph holds no memset nodes;
@@ -33,6 +35,8 @@ constexpr int BT = 256;  // threads per tile workgroup
 
 struct BArgs {
     int64_t n_tiles;
+    int64_t n_launch;          // tiles launched per sweep (the live ones)
+    const int32_t* tile_list;  // [n_launch] tile of each launched workgroup
     const int32_t* tile_sys;   // [n_tiles]
     const int64_t* tile_row;   // [n_tiles + 1] row range of each tile
     const int32_t* rowptr;
@@ -80,7 +84,7 @@ __device__ inline void block_sum(double (&v)[NR], double* out) {
 // over the row's columns, partial p_new . q.  G threads per row.
 template <int NR, int G>
 __global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
-    const int64_t t = blockIdx.x;
+    const int64_t t = a.tile_list[blockIdx.x];
     const int s = a.tile_sys[t];
     if (tile_done<NR>(a, s)) return;
     double beta[NR];
@@ -142,7 +146,7 @@ __global__ __launch_bounds__(256) void k_breduce_pq(BArgs a) {
 // x += alpha p, r -= alpha q, z = dinv r; partial r.z and r.r
 template <int NR>
 __global__ __launch_bounds__(BT) void k_bupdate(BArgs a) {
-    const int64_t t = blockIdx.x;
+    const int64_t t = a.tile_list[blockIdx.x];
     const int s = a.tile_sys[t];
     if (tile_done<NR>(a, s)) return;
     double alpha[NR];
@@ -342,11 +346,11 @@ Tiles make_tiles(const std::vector<int64_t>& sys_ptr, int64_t rows_per_tile) {
 
 struct BatchRun {
     BArgs a{};
-    int32_t *d_tile_sys = nullptr;
+    int32_t *d_tile_sys = nullptr, *d_tile_list = nullptr;
     int64_t *d_tile_row = nullptr, *d_sys_tile = nullptr;
     ~BatchRun() {
-        void* p[] = {d_tile_sys, d_tile_row, d_sys_tile, a.x, a.r, a.z, a.pold, a.pnew, a.q, a.tpart, a.tpart2, a.scal,
-                     a.done, a.active, (void*)a.dinv};
+        void* p[] = {d_tile_sys, d_tile_list, d_tile_row, d_sys_tile, a.x, a.r, a.z, a.pold, a.pnew, a.q, a.tpart,
+                     a.tpart2, a.scal, a.done, a.active, (void*)a.dinv};
         for (void* q : p)
             if (q) (void)hipFree(q);
     }
@@ -354,7 +358,7 @@ struct BatchRun {
 
 template <int NR, int G>
 void enqueue_iteration(BArgs& a, hipStream_t st, int maxit) {
-    const unsigned nt = (unsigned)a.n_tiles;
+    const unsigned nt = (unsigned)a.n_launch;
     const unsigned np = (unsigned)(((int64_t)a.n_sys * NR * 64 + 255) / 256);  // one wavefront per pair
     hipLaunchKernelGGL((k_bspmm<NR, G>), dim3(nt), dim3(BT), 0, st, a);
     hipLaunchKernelGGL(k_breduce_pq<NR>, dim3(np), dim3(256), 0, st, a);
@@ -408,9 +412,13 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
     for (int cc = 0; cc < NR; ++cc)
         for (int64_t r = 0; r < nf; ++r) bi[r * NR + cc] = b[(size_t)cc * nf + r];
     std::vector<double> r_host = bi;
+    Tiles T;
     auto set_tiles = [&](int rows) {
-        Tiles T = make_tiles(sys_ptr, rows);
-        if (run.d_tile_sys) { (void)hipFree(run.d_tile_sys); (void)hipFree(run.d_tile_row); (void)hipFree(run.d_sys_tile); }
+        T = make_tiles(sys_ptr, rows);
+        if (run.d_tile_sys) {
+            (void)hipFree(run.d_tile_sys); (void)hipFree(run.d_tile_row); (void)hipFree(run.d_sys_tile);
+            (void)hipFree(run.d_tile_list);
+        }
         if (a.tpart) { (void)hipFree(a.tpart); (void)hipFree(a.tpart2); }
         run.d_tile_sys = balloc<int32_t>(T.sys.size());
         run.d_tile_row = balloc<int64_t>(T.row.size());
@@ -419,6 +427,12 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
         BCHK(hipMemcpy(run.d_tile_row, T.row.data(), sizeof(int64_t) * T.row.size(), hipMemcpyHostToDevice));
         BCHK(hipMemcpy(run.d_sys_tile, T.sys_tile.data(), sizeof(int64_t) * T.sys_tile.size(), hipMemcpyHostToDevice));
         a.n_tiles = (int64_t)T.sys.size();
+        std::vector<int32_t> ident(T.sys.size());
+        for (size_t k = 0; k < ident.size(); ++k) ident[k] = (int32_t)k;
+        run.d_tile_list = balloc<int32_t>(ident.size());
+        BCHK(hipMemcpy(run.d_tile_list, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice));
+        a.tile_list = run.d_tile_list;
+        a.n_launch = a.n_tiles;
         a.tile_sys = run.d_tile_sys;
         a.tile_row = run.d_tile_row;
         a.sys_tile = run.d_sys_tile;
@@ -458,29 +472,52 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
     init();
     // ---- capture CHUNK iterations in one graph, replay until done ----
     const int CHUNK = 16;
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    BArgs a0 = a;  // the ping-pong state is the same after an even chunk
-    BCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < CHUNK; ++k) {
-        // the count of pairs still running is taken in the chunk's last iteration
-        if (k == CHUNK - 1) hipLaunchKernelGGL(k_bclear_active, dim3(1), dim3(1), 0, c.stream, a.active);
-        enqueue_iteration_g<NR>(bestG, a, c.stream, maxit);
-    }
-    BCHK(hipMemcpyAsync(h_active, a.active, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    BCHK(hipStreamEndCapture(c.stream, &graph));
-    BCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    a = a0;
+    int* h_done = nullptr;
+    BCHK(hipHostMalloc((void**)&h_done, sizeof(int) * (size_t)n_sys * NR, hipHostMallocDefault));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto capture = [&]() {
+        if (exec) { (void)hipGraphExecDestroy(exec); (void)hipGraphDestroy(graph); }
+        BArgs a0 = a;  // the ping-pong state is the same after an even chunk
+        BCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < CHUNK; ++k) {
+            // the running flag is taken in the chunk's last iteration
+            if (k == CHUNK - 1) hipLaunchKernelGGL(k_bclear_active, dim3(1), dim3(1), 0, c.stream, a.active);
+            enqueue_iteration_g<NR>(bestG, a, c.stream, maxit);
+        }
+        BCHK(hipMemcpyAsync(h_active, a.active, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        BCHK(hipMemcpyAsync(h_done, a.done, sizeof(int) * (size_t)n_sys * NR, hipMemcpyDeviceToHost, c.stream));
+        BCHK(hipStreamEndCapture(c.stream, &graph));
+        BCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        a = a0;
+    };
+    capture();
     hipEvent_t t0, t1;
     BCHK(hipEventCreate(&t0));
     BCHK(hipEventCreate(&t1));
     BCHK(hipEventRecord(t0, c.stream));
     int chunks = 0;
+    std::vector<int32_t> live;
     for (;;) {
         BCHK(hipGraphLaunch(exec, c.stream));
         BCHK(hipStreamSynchronize(c.stream));
         ++chunks;
         if (*h_active == 0 || chunks * CHUNK >= maxit + CHUNK) break;
+        // compact the launch list to the tiles of systems with a running pair
+        live.clear();
+        for (int s = 0; s < n_sys; ++s) {
+            bool run_s = false;
+            for (int cc = 0; cc < NR; ++cc) run_s = run_s || h_done[s * NR + cc] == 0;
+            if (run_s)
+                for (int64_t t = T.sys_tile[s]; t < T.sys_tile[s + 1]; ++t) live.push_back((int32_t)t);
+        }
+        if (!live.empty() && (double)live.size() < 0.6 * (double)a.n_launch) {
+            BCHK(hipMemcpyAsync(run.d_tile_list, live.data(), sizeof(int32_t) * live.size(), hipMemcpyHostToDevice,
+                                c.stream));
+            BCHK(hipStreamSynchronize(c.stream));
+            a.n_launch = (int64_t)live.size();
+            capture();
+        }
     }
     BCHK(hipEventRecord(t1, c.stream));
     BCHK(hipEventSynchronize(t1));
@@ -491,6 +528,7 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
     (void)hipEventDestroy(t1);
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
+    (void)hipHostFree(h_done);
     // results
     std::vector<double> xi((size_t)nf * NR), sc((size_t)n_sys * NR * S_N);
     BCHK(hipMemcpy(xi.data(), a.x, sizeof(double) * xi.size(), hipMemcpyDeviceToHost));
