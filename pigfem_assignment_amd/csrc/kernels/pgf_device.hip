@@ -262,19 +262,26 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t ng, const int32_t* __restr
     }
 }
 
-// PCG step A: p_new = z + beta p_old (own rows), q = A p_new (columns gathered
-// on the fly from z and p_old, once per node group), alpha = rz / (p.q) in the
-// last block.  The matrix may be stored in a narrower type than the Krylov
-// vectors.
+// PCG step A1: p = z + beta p (elementwise, in place).
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_pupd(int64_t n, const T* __restrict__ z, T* __restrict__ p,
+                                                 const double* scal, const int* done) {
+    if (*done) return;
+    const double beta = scal[S_BETA];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (T)((double)z[i] + beta * (double)p[i]);
+}
+
+// PCG step A2: q = A p by node groups (each column index and p entry loaded
+// once per group), per-block partial of p.q.  The matrix may be stored in a
+// narrower type than the Krylov vectors.
 template <typename TA, typename T, int G>
 __global__ __launch_bounds__(256) void k_cg_spmv(int64_t ng, const int32_t* __restrict__ grow,
                                                  const int32_t* __restrict__ rowptr,
                                                  const int32_t* __restrict__ rowsplit, const int32_t* __restrict__ col,
-                                                 const TA* __restrict__ A, const T* __restrict__ z,
-                                                 const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
-                                                 const double* scal, double* partial, const int* done) {
+                                                 const TA* __restrict__ A, const T* __restrict__ p, T* __restrict__ q,
+                                                 double* partial, const int* done) {
     if (*done) return;
-    const double beta = scal[S_BETA];
     const int lane = threadIdx.x % G;
     double dot = 0.0;
     for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; g < ng;
@@ -284,8 +291,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t ng, const int32_t* __re
         const int32_t b1 = k > 1 ? rowptr[r0 + 1] : b0, b2 = k > 2 ? rowptr[r0 + 2] : b0;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
         for (int32_t j = lane; j < len; j += G) {
-            const int32_t c = col[b0 + j];
-            const double pv = (double)z[c] + beta * (double)pold[c];
+            const double pv = (double)p[col[b0 + j]];
             a0 += (double)A[b0 + j] * pv;
             if (k > 1) a1 += (double)A[b1 + j] * pv;
             if (k > 2) a2 += (double)A[b2 + j] * pv;
@@ -298,10 +304,8 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t ng, const int32_t* __re
         if (lane == 0) {
             const double av[3] = {a0, a1, a2};
             for (int t = 0; t < k; ++t) {
-                const double pn = (double)z[r0 + t] + beta * (double)pold[r0 + t];
-                pnew[r0 + t] = (T)pn;
                 q[r0 + t] = (T)av[t];
-                dot += pn * av[t];
+                dot += (double)p[r0 + t] * av[t];
             }
         }
     }
@@ -481,16 +485,15 @@ static int run_pcg(Ctx& c, const TA* A, const void* b, bool b_is_double, T* x, T
     double last_report = wall_s();
     while (!stop) {
         for (int k = 0; k < chunk; ++k, ++it) {
-            T* pold = (it % 2 == 0) ? p0 : p1;
-            T* pnew = (it % 2 == 0) ? p1 : p0;
+            T* pnew = p0;  // p is updated in place
+            hipLaunchKernelGGL((k_cg_pupd<T>), dim3(gu), dim3(256), 0, c.stream, n, (const T*)z, pnew,
+                               (const double*)c.scal, (const int*)c.done);
             if (G == 16)
                 hipLaunchKernelGGL((k_cg_spmv<TA, T, 16>), dim3(ga), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr,
-                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, (const double*)c.scal, part_a,
-                                   (const int*)c.done);
+                                   c.rowsplit, c.colidx, A, (const T*)pnew, q, part_a, (const int*)c.done);
             else
                 hipLaunchKernelGGL((k_cg_spmv<TA, T, 8>), dim3(ga), dim3(256), 0, c.stream, c.n_grp, c.grp_row, c.rowptr,
-                                   c.rowsplit, c.colidx, A, z, pold, pnew, q, (const double*)c.scal, part_a,
-                                   (const int*)c.done);
+                                   c.rowsplit, c.colidx, A, (const T*)pnew, q, part_a, (const int*)c.done);
             hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c.stream, (int)ga, (const double*)part_a, c.scal,
                                (const int*)c.done);
             hipLaunchKernelGGL((k_cg_update<T>), dim3(gu), dim3(256), 0, c.stream, n, x, r, z, pnew, q, dinv,
