@@ -44,7 +44,7 @@ struct BArgs {
     const int32_t* colidx;
     const double* K;
     const double* dinv;        // [n_free]
-    double *x, *r, *z, *pold, *pnew, *q;  // [n_free][NR]
+    double *x, *r, *z, *pnew, *q;  // [n_free][NR]; pnew is p, updated in place
     double* tpart;             // [n_tiles][NR]
     double* tpart2;            // [n_tiles][2*NR]
     const int64_t* sys_tile;   // [n_sys + 1] tile range of each system
@@ -80,10 +80,9 @@ __device__ inline void block_sum(double (&v)[NR], double* out) {
     __syncthreads();
 }
 
-// p_new = z + beta p_old on the tile's rows (written), q = K (z + beta p_old)
-// over the row's columns, partial p_new . q.  G threads per row.
-template <int NR, int G>
-__global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
+// p = z + beta p on the tile's rows, in place, for the pairs still running
+template <int NR>
+__global__ __launch_bounds__(BT) void k_bpupd(BArgs a) {
     const int64_t t = a.tile_list[blockIdx.x];
     const int s = a.tile_sys[t];
     if (tile_done<NR>(a, s)) return;
@@ -93,6 +92,20 @@ __global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
         beta[c] = a.scal[(s * NR + c) * S_N + S_BETA];
         act[c] = a.done[s * NR + c] == 0;
     }
+    for (int64_t r = a.tile_row[t] + threadIdx.x; r < a.tile_row[t + 1]; r += BT)
+        for (int c = 0; c < NR; ++c)
+            if (act[c]) a.pnew[r * NR + c] = a.z[r * NR + c] + beta[c] * a.pnew[r * NR + c];
+}
+
+// q = K p over the row's columns (p gathered once per column for all NR
+// right-hand sides), partial p . q.  G threads per row.
+template <int NR, int G>
+__global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
+    const int64_t t = a.tile_list[blockIdx.x];
+    const int s = a.tile_sys[t];
+    if (tile_done<NR>(a, s)) return;
+    bool act[NR];
+    for (int c = 0; c < NR; ++c) act[c] = a.done[s * NR + c] == 0;
     const int64_t r0 = a.tile_row[t], r1 = a.tile_row[t + 1];
     const int sub = threadIdx.x % G;
     double pq[NR];
@@ -104,18 +117,15 @@ __global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
         for (int32_t j = b + sub; j < e; j += G) {
             const double v = a.K[j];
             const int64_t col = a.colidx[j];
-            for (int c = 0; c < NR; ++c)
-                acc[c] += v * (a.z[col * NR + c] + beta[c] * a.pold[col * NR + c]);
+            for (int c = 0; c < NR; ++c) acc[c] += v * a.pnew[col * NR + c];
         }
         for (int c = 0; c < NR; ++c)
             for (int m = G / 2; m >= 1; m >>= 1) acc[c] += __shfl_xor(acc[c], m, G);
         if (sub == 0)
             for (int c = 0; c < NR; ++c) {
                 if (!act[c]) continue;
-                const double pn = a.z[r * NR + c] + beta[c] * a.pold[r * NR + c];
-                a.pnew[r * NR + c] = pn;
                 a.q[r * NR + c] = acc[c];
-                pq[c] += pn * acc[c];
+                pq[c] += a.pnew[r * NR + c] * acc[c];
             }
     }
     double tot[NR];
@@ -229,7 +239,6 @@ __global__ __launch_bounds__(BT) void k_binit(BArgs a) {
             const double zn = di * rn;
             a.x[i] = 0.0;
             a.z[i] = zn;
-            a.pold[i] = 0.0;
             a.pnew[i] = 0.0;
             rz[c] += rn * zn;
             rr[c] += rn * rn;
@@ -349,7 +358,7 @@ struct BatchRun {
     int32_t *d_tile_sys = nullptr, *d_tile_list = nullptr;
     int64_t *d_tile_row = nullptr, *d_sys_tile = nullptr;
     ~BatchRun() {
-        void* p[] = {d_tile_sys, d_tile_list, d_tile_row, d_sys_tile, a.x, a.r, a.z, a.pold, a.pnew, a.q, a.tpart,
+        void* p[] = {d_tile_sys, d_tile_list, d_tile_row, d_sys_tile, a.x, a.r, a.z, a.pnew, a.q, a.tpart,
                      a.tpart2, a.scal, a.done, a.active, (void*)a.dinv};
         for (void* q : p)
             if (q) (void)hipFree(q);
@@ -360,11 +369,11 @@ template <int NR, int G>
 void enqueue_iteration(BArgs& a, hipStream_t st, int maxit) {
     const unsigned nt = (unsigned)a.n_launch;
     const unsigned np = (unsigned)(((int64_t)a.n_sys * NR * 64 + 255) / 256);  // one wavefront per pair
+    hipLaunchKernelGGL(k_bpupd<NR>, dim3(nt), dim3(BT), 0, st, a);
     hipLaunchKernelGGL((k_bspmm<NR, G>), dim3(nt), dim3(BT), 0, st, a);
     hipLaunchKernelGGL(k_breduce_pq<NR>, dim3(np), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_bupdate<NR>, dim3(nt), dim3(BT), 0, st, a);
     hipLaunchKernelGGL(k_breduce_rz<NR>, dim3(np), dim3(256), 0, st, a, maxit);
-    std::swap(a.pold, a.pnew);
 }
 
 template <int NR>
@@ -399,8 +408,7 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
     a.x = balloc<double>(nf * NR);
     a.r = balloc<double>(nf * NR);
     a.z = balloc<double>(nf * NR);
-    a.pold = balloc<double>(nf * NR);
-    a.pnew = balloc<double>(nf * NR);
+    a.pnew = balloc<double>(nf * NR);  // p, updated in place
     a.q = balloc<double>(nf * NR);
     a.scal = balloc<double>((size_t)n_sys * NR * S_N);
     a.done = balloc<int>((size_t)n_sys * NR);
@@ -478,7 +486,7 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
     hipGraphExec_t exec = nullptr;
     auto capture = [&]() {
         if (exec) { (void)hipGraphExecDestroy(exec); (void)hipGraphDestroy(graph); }
-        BArgs a0 = a;  // the ping-pong state is the same after an even chunk
+        BArgs a0 = a;
         BCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < CHUNK; ++k) {
             // the running flag is taken in the chunk's last iteration
