@@ -48,6 +48,7 @@ def lib():
         L.orc_detect_nodes.argtypes = [C.c_int64, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_law.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_void_p]
         L.orc_sample_range.argtypes = [C.c_int64, C.c_int64]
+        L.orc_shape.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_edge_root.restype = C.c_double
         L.orc_edge_root.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -183,3 +184,17 @@ def edge_root(p0, p1, incl):
     p1 = np.ascontiguousarray(p1, np.float64)
     incl = np.ascontiguousarray(incl, np.float64)
     return lib().orc_edge_root(_p(p0), _p(p1), _p(incl))
+
+
+def shape(dim, nn, xi):
+    """Parent shape values [nn] and gradients [nn][dim] at parent point xi;
+    for the Wachspress n-gon (dim 2, nn >= 5) also its reference vertices
+    [nn][2] and fan rule [4 nn][3] (xi, eta, weight)."""
+    x = np.ascontiguousarray(xi, np.float64)
+    N = np.zeros(nn)
+    dN = np.zeros((nn, dim))
+    vtx = np.zeros((nn, 2))
+    rule = np.zeros((4 * nn, 3))
+    poly = dim == 2 and nn >= 5
+    lib().orc_shape(dim, nn, _p(x), _p(N), _p(dN), _p(vtx) if poly else None, _p(rule) if poly else None)
+    return (N, dN, vtx, rule) if poly else (N, dN)

@@ -881,6 +881,33 @@ ORC_API void orc_law(const double* law, int dim, const double* delta_in, double 
     out[dim + dim * dim + 1] = fl;
 }
 
+// shape functions of a volumetric parent element at a parent point: dim 3 the
+// linear tet, dim 2 with nn = 3 the triangle, nn = 4 the bilinear quad and
+// nn >= 5 the Wachspress n-gon on its regular reference polygon.  Also returns
+// the polygon's reference vertices (vtx [nn][2], nn >= 5 only) and, for
+// nn >= 5, the points and weights of its fan rule (rule [4 nn][3]: xi, eta, w).
+ORC_API void orc_shape(int dim, int nn, const double* xi, double* N, double* dN, double* vtx, double* rule) {
+    Vec x(xi, xi + dim), Nv;
+    Mat dNv;
+    shape(dim, nn, x, Nv, dNv);
+    for (int k = 0; k < nn; ++k) {
+        N[k] = Nv[k];
+        for (int i = 0; i < dim; ++i) dN[k * dim + i] = dNv[k][i];
+    }
+    if (dim == 2 && nn >= 5) {
+        if (vtx)
+            for (int k = 0; k < nn; ++k) poly_vertex(nn, k, vtx + 2 * k);
+        if (rule) {
+            Rule r = rule_poly(nn);
+            for (size_t q = 0; q < r.w.size(); ++q) {
+                rule[3 * q] = r.xi[q][0];
+                rule[3 * q + 1] = r.xi[q][1];
+                rule[3 * q + 2] = r.w[q];
+            }
+        }
+    }
+}
+
 // --------- integer flag checks: node detection and enrichment-node set -------
 // inclusions: [n][5] = type, cx, cy, cz, r.  coords [3][n_nodes] are the
 // ORIGINAL (unmoved) coordinates; first_elem_node0/1 give the "other" node of
