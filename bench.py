@@ -1,16 +1,24 @@
 """Benchmark of the PIGFEM assemble-and-solve path on one MI355X.
 
-Workload (default): 2D unit square, 1000 x 1000 QUAD4, 400 seeded circular
-inclusions (RSA, radii U[0.005, 0.02]), matrix E=3.2 GPa nu=0.35, inclusion
-E=70 GPa nu=0.22, plane strain, left edge fixed, right edge u_x=0.01.
+Workload (default, the 3D cohesive configuration): unit cube, 128^3 cubes x 6
+TET4 (12.6 M elements), 200 seeded spheres (RSA, radii U[0.03, 0.06]) with
+cohesive interfaces (bilinear law, sigma_c = 30 MPa, delta_c = 1e-3,
+delta_f = 1e-2), matrix E=3.2 GPa nu=0.35, inclusion E=70 GPa nu=0.22, left
+face fixed, right face u_x = 0.02 applied in 20 equal increments.
 
 A step is one full FP64 Jacobian + internal-force assembly over every element
-(the reference's assemble_nonlinear(func=true, jac=true)), scattered into the
-CSR matrix with the reference's sparsity pattern; inputs are resident in HBM.
-value = elements assembled per second.  The same JSON line also carries the
-time to a 1e-8 residual for the load step (plain FP64 PCG vs FP32 PCG with
-FP64 residual), SpMV bandwidth, the roofline of the dominant kernel and a CPU
-baseline (the oracle's reference-structured element loop on one host core).
+(the reference's assemble_nonlinear(func=true, jac=true)), cohesive law and
+damage update included, scattered into the CSR matrix with the reference's
+sparsity pattern; inputs are resident in HBM.  value = elements assembled per
+second.  The same JSON line carries the roofline of the dominant assembly
+kernel (and of the other pass), the time to a 1e-8 FP64 residual for the
+first two load increments in four modes (plain FP64 PCG; previous increment
+only; learned network + previous increment; the same with FP32 matrix
+values), each mode's displacement error against the plain FP64 solve, the
+time-to-residual roofline fraction, SpMV bandwidth and a CPU baseline (the
+oracle's reference-structured element loop and its Jacobi-PCG, timed on this
+host on 1 core and on all cores).  --workload c1 / c3 run the 2D single-mesh
+and the batched-RVE workloads instead.
 """
 import argparse
 import json
@@ -68,13 +76,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-learn", action="store_true", help="skip the learned-guess load steps")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c2")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device five times (calibrates the HBM byte counters)")
     ap.add_argument("--c2-n", type=int, default=128)
     ap.add_argument("--c2-incl", type=int, default=200)
     ap.add_argument("--c2-steps", type=int, default=2, help="load increments timed (of 20 to 2%% strain)")
-    ap.add_argument("--c2-modes", default="plain_fp64,learned_fp64_forcing,learned_fp32_forcing",
+    ap.add_argument("--c2-modes", default="plain_fp64,prev_fp64_forcing,learned_fp64_forcing,learned_fp32_forcing",
                     help="comma list of " + ",".join(ALL_MODES))
     ap.add_argument("--c3-rves", type=int, default=256)
     ap.add_argument("--c3-nmin", type=int, default=64)
@@ -204,6 +212,7 @@ def main():
 # refines, as in iterative refinement, down to the 1e-8 target; a solve that
 # has not converged after 2000 FP32 iterations continues in FP64.
 ALL_MODES = {"plain_fp64": (0, 64, 1e-10, False, 0), "plain_fp64_forcing": (0, 64, 1e-10, True, 0),
+             "prev_fp64_forcing": (3, 64, 1e-10, True, 0),
              "learned_fp64_forcing": (2, 64, 1e-10, True, 0), "learned_fp32_forcing": (2, 32, 1e-3, True, 2000)}
 
 
@@ -219,10 +228,12 @@ def copy_probe():
     print("copied 5 x %d bytes" % (4 * n), flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per launch of the gather pass from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
-    f = os.path.join(ROOT, "profiles", "assembly_c1_hbm_traffic.txt")
+def measured_traffic(workload="c1", kernel="k_gather"):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    f = os.path.join(ROOT, "profiles", "%s_%s_hbm_traffic.txt" % (kernel, workload))
+    if workload == "c1" and kernel == "k_gather":
+        f = os.path.join(ROOT, "profiles", "assembly_c1_hbm_traffic.txt")
     if not os.path.exists(f):
         return None
     for line in open(f):
@@ -232,13 +243,71 @@ def measured_traffic():
     return None
 
 
+def cut_pass_bytes(prob):
+    """Algorithmic bytes of one cut-element pass (k_cut_elem): per cut element
+    its integration-point tables (gradients of every slot, J*W, material),
+    per cohesive point N, R, dA*W, node map, law, delta_max read and written
+    and the flags written, the element's displacements read, and its K_e and
+    P_e written once (FP64)."""
+    d = prob.desc()
+    nc = prob.info.n_cut
+    if nc == 0:
+        return 0
+    dim = prob.dim
+    ne = np.ctypeslib.as_array(d.cut_n_enrich, shape=(nc,)).astype(np.int64)
+    vp = np.ctypeslib.as_array(d.vpt_ptr, shape=(nc + 1,)).astype(np.int64)
+    cp = np.ctypeslib.as_array(d.cpt_ptr, shape=(nc + 1,)).astype(np.int64)
+    M = dim * (4 + ne)
+    nv, ncp = np.diff(vp), np.diff(cp)
+    per = nv * (8 * M + 8 + 4) + ncp * (4 * 8 + 9 * 8 + 8 + 8 + 1 + 4 + 8 + 8 + 1) + 8 * M + 8 * M * M + 8 * M
+    return int(per.sum())
+
+
+def cut_scratch_bytes(prob):
+    """K_e and P_e of every cut element, written by the cut pass and read once
+    by the gather pass"""
+    nc = prob.info.n_cut
+    if nc == 0:
+        return 0
+    d = prob.desc()
+    ne = np.ctypeslib.as_array(d.cut_n_enrich, shape=(nc,)).astype(np.int64)
+    M = prob.dim * (4 + ne)
+    return int((8 * M * M + 8 * M).sum())
+
+
+def pcg_iteration_bytes(nnz_ff, n, value_bytes=8):
+    """Algorithmic bytes of one Jacobi-PCG iteration: SpMV (values, int32
+    column indices, row pointers, x read once, y written) plus two dot
+    products, three axpys and the preconditioner as separate vector passes
+    (the unfused sum; the fused kernels are reported against it)."""
+    return (value_bytes + 4) * nnz_ff + 4 * (n + 1) + 16 * n + 128 * n
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail
+
+
 def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, train_steps=5, epochs=30, log=print,
-                     ctx=None, modes=tuple(ALL_MODES)):
-    """Time-to-1e-8 residual per load step of a 3D cohesive inclusion problem:
-    plain FP64 PCG from a zero start vs the learned start with the FP32 inner
-    solve (FP64 residual), and the learned start with FP64.  The network is
-    trained first on plain solves of two smaller problems (its voxel grid does
-    not depend on the mesh)."""
+                     ctx=None, modes=tuple(ALL_MODES), spmv_info=None):
+    """Time-to-1e-8 residual per load step of a 3D cohesive inclusion problem
+    in several modes: plain FP64 PCG from a zero start; the Galerkin-scaled
+    previous increment only (the method without the network); the network's
+    columns combined with the previous increment; the same with FP32 matrix
+    values.  The network is trained first on plain solves of two smaller
+    problems (its voxel grid does not depend on the mesh).  Every mode's
+    displacement after every increment is compared with the plain FP64 one."""
     from pigfem_assignment_amd import learn, configs
     from pigfem_assignment_amd.api import GuessNet
     t0 = time.time()
@@ -252,33 +321,77 @@ def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, tra
     out = {"learn_train_s": t_train, "learn_loss_first": float(loss[0]), "learn_loss_last": float(loss[-1]),
            "learn_workload": "unit cube %d^3 x 6 TET4, %d spheres, bilinear cohesive, first %d of 20 increments" % (
                n, n_incl, steps)}
-    # plain: zero start, PCG to 1e-10 in every Newton iteration; the others use
-    # the inexact-Newton forcing term (each solve only as tight as needed)
     modes = [(m,) + ALL_MODES[m] for m in modes]
-    flags = {}
+    flags, sols = {}, {}
     for name, ug, prec, lin, adapt, m32 in modes:
+        U_steps = []
         recs, fl, ctx = learn.run_steps(prob[0], prob[1], steps, dfrac, use_guess=ug, net=net, lin_rtol=lin,
-                                        precision=prec, ctx=ctx, adaptive_lin=adapt, fp32_maxit=m32)
+                                        precision=prec, ctx=ctx, adaptive_lin=adapt, fp32_maxit=m32,
+                                        keep_solutions=U_steps)
         flags[name] = fl
+        sols[name] = U_steps
         out["%s_ms_per_step" % name] = [r["ms"] for r in recs]
         out["%s_pcg_per_step" % name] = [r["pcg"] for r in recs]
         out["%s_newton_per_step" % name] = [r["newton"] for r in recs]
+        out["%s_res_rel_per_step" % name] = [r["res"] / r["res0"] if r["res0"] else 0.0 for r in recs]
+        if spmv_info is not None:
+            nnz_ff, nf, b_asm = spmv_info
+            fr = []
+            for r in recs:
+                b = r["pcg"] * pcg_iteration_bytes(nnz_ff, nf, 4 if prec == 32 else 8) + (r["newton"] + 1) * b_asm
+                fr.append(b / (r["ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9))
+            out["%s_time_to_residual_roofline_frac" % name] = fr
         if prec == 32:
             out["%s_fp64_fallback_per_step" % name] = [r["fp64_fallback"] for r in recs]
         log("# %s: ms per step %s, PCG %s, Newton %s%s" % (
             name, ["%.1f" % r["ms"] for r in recs], [r["pcg"] for r in recs], [r["newton"] for r in recs],
             ", FP64 fallback at Newton iteration %s" % [r["fp64_fallback"] for r in recs] if prec == 32 else ""))
+    ref_name = "plain_fp64" if "plain_fp64" in sols else modes[0][0]
+    for name in sols:
+        errs = [float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(sols[name], sols[ref_name])]
+        out["%s_U_rel_err_vs_%s" % (name, ref_name)] = errs
+        log("# %s: displacement error vs %s per step %s" % (name, ref_name, ["%.2e" % e for e in errs]))
     ref = flags[modes[0][0]]
     same = all(np.array_equal(a, b) for m in flags for a, b in zip(ref, flags[m]))
     out["damage_flags_identical"] = bool(same)
     tot = {m[0]: sum(out["%s_ms_per_step" % m[0]]) for m in modes}
-    for a, b in (("plain_fp64", "learned_fp32_forcing"), ("plain_fp64_forcing", "learned_fp32_forcing"),
-                 ("plain_fp64", "learned_fp64_forcing"), ("plain_fp64_forcing", "learned_fp64_forcing")):
+    for a, b in (("plain_fp64", "learned_fp32_forcing"), ("plain_fp64", "learned_fp64_forcing"),
+                 ("plain_fp64", "prev_fp64_forcing"), ("prev_fp64_forcing", "learned_fp64_forcing"),
+                 ("prev_fp64_forcing", "learned_fp32_forcing")):
         if a in tot and b in tot:
             out["speedup_%s_vs_%s" % (b, a)] = tot[a] / tot[b]
             log("# %s vs %s: %.2fx" % (b, a, tot[a] / tot[b]))
     log("# damage flags identical across modes: %s" % same)
     return out
+
+
+def cpu_baselines(prob, ctx, n_sample, log, threads):
+    """The oracle's reference-structured element loop on a bounded sample of
+    elements (1 core and `threads` cores), and its Jacobi-PCG iteration on the
+    device-assembled K_ff (1 core and `threads` cores)."""
+    from oracle import oracle as orc
+    info = prob.info
+    o = orc.Oracle(prob, prob.bc_list(), sample=(0, n_sample))
+    n_cut_sample = int(np.sum(prob.cut()[:, 0] < n_sample))
+    _, _, ms1 = o.assemble(dt=0.05)
+    orc.set_threads(threads)
+    _, _, msN = o.assemble(dt=0.05)
+    orc.set_threads(1)
+    del o
+    log("# cpu oracle assembly of %d elements (%d cut): %.1f ms on 1 core, %.1f ms on %d cores" % (
+        n_sample, n_cut_sample, ms1, msN, threads))
+    rp, rs, ci = ctx.pattern()
+    K = ctx.matrix()
+    nf = info.n_free
+    b = np.ones(nf)
+    it1, itN = 3, 10
+    pcg1 = orc.pcg_ms_per_iter(rp[: nf + 1], rs[:nf], ci, K, b, it1, 1)
+    pcgN = orc.pcg_ms_per_iter(rp[: nf + 1], rs[:nf], ci, K, b, itN, threads)
+    log("# cpu Jacobi-PCG iteration on the same K_ff: %.1f ms on 1 core (%d iterations), %.1f ms on %d cores "
+        "(%d iterations)" % (pcg1, it1, pcgN, threads, itN))
+    return {"assembly_elem_per_s_1core": n_sample / (ms1 * 1e-3), "assembly_elem_per_s_all": n_sample / (msN * 1e-3),
+            "assembly_sample_elements": n_sample, "assembly_sample_cut": n_cut_sample,
+            "pcg_ms_per_iteration_1core": pcg1, "pcg_ms_per_iteration_all": pcgN, "threads_all": threads}
 
 
 def bench_c2(args):
@@ -288,22 +401,36 @@ def bench_c2(args):
     prob = configs.c2_problem(n=args.c2_n, n_incl=args.c2_incl)
     incl = configs.c2_inclusions(args.c2_incl, args.c2_n)
     info = prob.info
+    setup_s = time.time() - t0
     print("# C2 setup: %d tets (%d cut), %d dofs, %d cohesive points, host %.1f s" % (
-        info.n_elem, info.n_cut, info.n_free, info.n_cpt, time.time() - t0), flush=True)
+        info.n_elem, info.n_cut, info.n_free, info.n_cpt, setup_s), flush=True)
     ctx = DeviceContext(prob)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
     slots, slot_ms = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags, dt=0.05)
     ms = ctx.time_assemble(flags, reps=args.steps)
+    value = info.n_elem / (ms * 1e-3)
     ms_call, ms_cut, ms_gather = ctx.time_assemble_split(flags, reps=args.steps)
     nnz_free_rows = int(ctx.pattern()[0][info.n_free])
-    bytes_gather = gather_bytes(3, nnz_free_rows, info.n_free, info.n_all_nodes, info.n_elem)
-    achieved = bytes_gather / (ms_gather * 1e-3) / 1e9
-    print("# C2 assembly K+f: %.2f ms (cut pass %.2f, gather %.2f), %.1f GB/s" % (ms, ms_cut, ms_gather, achieved),
-          flush=True)
-    extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather,
-             "gather_layout": slots, "gather_pass_ms_by_layout": slot_ms}
+    scratch = cut_scratch_bytes(prob)
+    bytes_gather = gather_bytes(3, nnz_free_rows, info.n_free, info.n_all_nodes, info.n_elem) + scratch
+    bytes_cut = cut_pass_bytes(prob)
+    gb_gather = bytes_gather / (ms_gather * 1e-3) / 1e9
+    gb_cut = bytes_cut / (ms_cut * 1e-3) / 1e9
+    print("# C2 assembly K+f FP64: %.2f ms (cut pass %.2f ms, %.0f GB/s; gather pass %.2f ms, %.0f GB/s), "
+          "%.3e elem/s" % (ms, ms_cut, gb_cut, ms_gather, gb_gather, value), flush=True)
+    if ms_cut >= ms_gather:
+        dom = ("k_cut_elem", gb_cut, bytes_cut, ms_cut)
+        other = {"kernel": "k_gather", "achieved": gb_gather, "frac": gb_gather / HBM_PEAK_GBS, "ms": ms_gather,
+                 "algorithmic_bytes": bytes_gather}
+    else:
+        dom = ("k_gather", gb_gather, bytes_gather, ms_gather)
+        other = {"kernel": "k_cut_elem", "achieved": gb_cut, "frac": gb_cut / HBM_PEAK_GBS, "ms": ms_cut,
+                 "algorithmic_bytes": bytes_cut}
+    extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather, "cut_pass_bytes": bytes_cut,
+             "gather_pass_bytes": bytes_gather, "gather_layout": slots, "gather_pass_ms_by_layout": slot_ms,
+             "roofline_other_pass": other, "host_setup_s": setup_s, "cohesive_points": info.n_cpt}
     # K_ff SpMV of the Newton solves: FP64 values, and FP32 values with FP64 vectors
     ctx.assemble(L.PGF_ASM_JAC, dt=0.05)
     ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FP32, dt=0.05)
@@ -315,20 +442,44 @@ def bench_c2(args):
         extra["spmv_fp%d_GBs" % prec] = gbs
         print("# C2 SpMV K_ff (%d nonzeros) %s: %.3f ms, %.0f GB/s" % (
             nnz_ff, "FP64" if prec == 64 else "FP32 values / FP64 vectors", t, gbs), flush=True)
+    cpu = None
+    if not args.no_cpu:
+        model, avail = cpu_info()
+        threads = max(1, min(16, avail))
+        c = cpu_baselines(prob, ctx, min(info.n_elem, 100_000), lambda m: print(m, flush=True), threads)
+        c.update({"cpu_model": model, "cpus_available": avail})
+        extra["cpu"] = c
+        cpu = {"value": c["assembly_elem_per_s_1core"], "unit": "element assemblies/s", "cores": 1, "kind": "port",
+               "sample": "K+f assembly of elements [0, %d) (%d cut) of the same 128^3 x 6 TET4 cohesive mesh by "
+                         "the CPU oracle: reference-structured element loop, per-QP dense kernels, map-based "
+                         "MatSetValue scatter, one core of %s; %d-core figures and the CPU Jacobi-PCG iteration "
+                         "time on the same K_ff are in extra.cpu" % (
+                             c["assembly_sample_elements"], c["assembly_sample_cut"], model, threads)}
     if not args.no_learn:
+        b_asm = bytes_gather + bytes_cut
         extra.update(learned_vs_plain(args.c2_n, args.c2_incl, args.c2_steps, prob=(prob, incl),
-                                      log=lambda m: print(m, flush=True), ctx=ctx, modes=args.c2_modes.split(",")))
+                                      log=lambda m: print(m, flush=True), ctx=ctx, modes=args.c2_modes.split(","),
+                                      spmv_info=(nnz_ff, nf, b_asm)))
+        if cpu is not None and "plain_fp64_pcg_per_step" in extra:
+            c = extra["cpu"]
+            est = [p * c["pcg_ms_per_iteration_all"] for p in extra["plain_fp64_pcg_per_step"]]
+            c["plain_fp64_time_to_residual_ms_estimate_all"] = est
+            print("# cpu plain FP64 time to 1e-8 per increment (GPU PCG iteration counts x CPU ms per iteration "
+                  "on %d cores): %s ms" % (c["threads_all"], ["%.0f" % e for e in est]), flush=True)
     line = {
         "metric": "element assemblies/s; time-to-1e-8 residual per load step, learned vs plain",
-        "value": info.n_elem / (ms * 1e-3), "unit": "element assemblies/s", "n_gpus": 1, "steps": args.steps,
+        "value": value, "unit": "element assemblies/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "3D unit cube %d^3 x 6 TET4, %d RSA spheres r~U[0.03,0.06], bilinear cohesive "
-                               "interfaces, K+f assembly and Newton load steps" % (args.c2_n, args.c2_incl),
-                   "elements": info.n_elem, "cut_elements": info.n_cut, "dofs": info.n_free},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-        "cpu_baseline": None,
+                               "interfaces (sigma_c 30 MPa, delta_c 1e-3, delta_f 1e-2), FP64 K+f assembly; load "
+                               "increments of 0.1%% strain to a 1e-8 residual" % (args.c2_n, args.c2_incl),
+                   "elements": info.n_elem, "cut_elements": info.n_cut, "dofs": info.n_free,
+                   "cohesive_points": info.n_cpt, "nnz_ff": nnz_ff},
+        "roofline": {"bound": "hbm", "achieved": dom[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom[1] / HBM_PEAK_GBS, "traffic": measured_traffic("c2", dom[0]),
+                     "kernel": dom[0], "algorithmic_bytes": dom[2], "ms": dom[3]},
+        "cpu_baseline": cpu,
         "extra": extra,
     }
     print(json.dumps(line))

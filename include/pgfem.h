@@ -13,13 +13,16 @@
  *   pgf_create / pgf_get_csr       Solver::preallocate_matrix + populate_local_neighbors
  *                                  (src/Solver.cpp:365-441, 723-829) and
  *                                  DofObject::get_elem_global_dofs (src/DofObject.cpp:189-211)
- *   pgf_assemble                   Assembler::assemble_linear / assemble_nonlinear
- *                                  (src/Assembler.cpp:47-184, 200-279, include/Assembler.h:41-51)
+ *   pgf_assemble                   Assembler::assemble_linear / assemble_nonlinear with their
+ *                                  element loops assemble_elem_linear / assemble_elem_nonlinear
+ *                                  (src/Assembler.cpp:48-93, 94-207, 208-304, 305-553,
+ *                                  include/Assembler.h:41-51) and the scatter set_matrix / set_vec
+ *                                  (src/Assembler.cpp:811-879)
  *                                  with KernelLinear / KernelNonlinear / KernelCohesive
  *                                  (src/AssemblerLinearElasticity.cpp:25-38,
  *                                  src/AssemblerSmallStrainNonlinear.cpp:28-91)
  *   pgf_set_solution               Solver::store_solution (src/Solver.cpp:860-973)
- *   pgf_set_load                   Assembler::assemble_new_load_step (src/Assembler.cpp:531-592)
+ *   pgf_set_load                   Assembler::assemble_new_load_step (src/Assembler.cpp:554-666)
  *   pgf_isv_commit / _rollback     InternalVars::copy_data_in / copy_data (src/InternalVars.cpp:43-86)
  *   pgf_spmv                       MatMult(K_ff, ...) (src/SolverLinear.cpp:128)
  *   pgf_pcg                        KSPSolve with KSPCG + PCJACOBI (the commented-out choice at
@@ -170,6 +173,10 @@ PGF_API int pgf_get_colors(pgf_ctx* ctx, int32_t* color);
 /* nodal displacement of every node, [dim][n_all_nodes] */
 PGF_API int pgf_set_solution(pgf_ctx* ctx, const double* u_nodal);
 PGF_API int pgf_get_solution(pgf_ctx* ctx, double* u_nodal);
+/* nodal displacement from a dof vector: free part x_free [n_free], prescribed
+ * part u_pres [n_pres] (NULL: zero); nodes sharing a dof (periodic images) all
+ * take its value (the reference's store_solution, src/Solver.cpp:860-973) */
+PGF_API int pgf_set_solution_dofs(pgf_ctx* ctx, const double* x_free, const double* u_pres);
 /* prescribed values (full load) [n_pres] and external free-dof force [n_free] (may be NULL) */
 PGF_API int pgf_set_load(pgf_ctx* ctx, const double* u_pres_full, const double* f_ext_full);
 /* apply load fraction t/T: prescribed dofs of the nodal field <- frac*u_pres_full */
@@ -218,6 +225,43 @@ PGF_API int pgf_rhs_prescribed(pgf_ctx* ctx, int32_t nrhs, const double* up, dou
  * assembly with PGF_ASM_ALL_ROWS; uf [nrhs][n_free], up [nrhs][n_pres], R [nrhs][n_pres] */
 PGF_API int pgf_reactions(pgf_ctx* ctx, int32_t nrhs, const double* uf, const double* up, double* R);
 
+/* homogenised response of RVEs at the current nodal solution u (the periodic
+ * fluctuation) and macro strain eps_bar [3 or 6] (Voigt, engineering shear):
+ * for every element segment k = [seg_elem[k], seg_elem[k+1]) the volume averages
+ * of sigma = D (eps(u) + eps_bar) and of eps(u) over its volumetric integration
+ * points, and the volume sum J*W.  out [n_seg][2*S+1].  Replaces
+ * SubscaleModel::homogenize_stress_strain (src/SubscaleModel.cpp:42-196). */
+PGF_API int pgf_volume_average(pgf_ctx* ctx, const double* eps_bar, int32_t n_seg, const int64_t* seg_elem,
+                               double* out);
+
+/* Allocation-free variant of pgf_volume_average for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference.  The staging of eps_bar and the segment
+ * list, the per-segment results and the partial sums live in one static
+ * device array per GPU (about 6.9 MB, held until the process ends), so
+ * n_seg is limited to PGF_NOALLOC_MAX_SEG (PGF_ELIMIT beyond it).  Calls that
+ * overlap on one GPU share that array and race; issue them from one host
+ * thread.  Same results as pgf_volume_average, bit for bit. */
+#define PGF_NOALLOC_MAX_SEG 1024
+PGF_API int pgf_volume_average_noalloc(pgf_ctx* ctx, const double* eps_bar, int32_t n_seg, const int64_t* seg_elem,
+                                       double* out);
+
+/* ---- shape sensitivities (direct method, SensitivityDirect::solveProblem,
+ * src/SensitivityDirect.cpp:25-188, with SensitivityShapeParameterStructural,
+ * src/SensitivityShapeParameterStructural.cpp:18-61) ----
+ * Tables from pgfh_shape_sensitivity (or their concatenation for a batch);
+ * param [n_cut] is the parameter slot of each cut element (-1: none). */
+PGF_API int pgf_set_shape_sensitivity(pgf_ctx* ctx, const double* dgrad, const double* divv, const int32_t* param);
+/* At the current nodal solution and macro strain eps_bar [S] (zero for a
+ * problem without body load): rhs [n_free] = -(dP_int/dr - dF_body/dr) of
+ * parameter slot `slot` (the direct method's right-hand side, K du/dr = rhs);
+ * dsig [n_seg][S] = integral over each element segment of the explicit
+ * stress derivative D dB u + sigma div v (the homogenised-stress sensitivity
+ * is (integral of D B du/dr + dsig) / volume).  rhs may be NULL; n_seg may
+ * be 0. */
+PGF_API int pgf_shape_pseudo_load(pgf_ctx* ctx, const double* eps_bar, int32_t slot, double* rhs, int32_t n_seg,
+                                  const int64_t* seg_elem, double* dsig);
+
 /* ---- Newton load step on the device (residual FP64) ---- */
 PGF_API int pgf_newton_step(pgf_ctx* ctx, double frac, const pgf_newton_opts* opts, pgf_step_info* info);
 
@@ -230,7 +274,7 @@ PGF_API int pgf_time_assemble_split(pgf_ctx* ctx, uint32_t flags, int32_t reps, 
 PGF_API int pgf_time_spmv(pgf_ctx* ctx, int32_t precision, int32_t reps, double* ms_per_call);
 
 /* ---- tuning of the row-gather assembly (no counterpart in the reference:
- * the reference assembles with MatSetValues, src/Assembler.cpp:47-184) ----
+ * the reference assembles with MatSetValue, src/Assembler.cpp:811-832) ----
  * slots: threads that share one unit's incident elements, 1, 2, 4 or 8;
  * node_rows: 0 = a unit is one matrix row, 1 = a unit is all rows of a node.
  * Default: 1 slot, one row per unit in 2D; 2 slots in 3D.  Rebuilds the

@@ -55,6 +55,12 @@ typedef struct pgfh_config {
     const int32_t* bc_node;     /* node id when bc_set == PGFH_SET_NODE */
     const int32_t* bc_dof;
     const double* bc_val;
+    /* periodic RVE (Mesh::build_periodic_nodesets, src/Mesh.cpp:603-741): the
+     * nodes on the upper faces share the dofs of their images on the lower
+     * faces, corners share node 0's; only the primary node's Dirichlet
+     * conditions count (DofObject::set_local_owned_normal_dofs,
+     * src/DofObject.cpp:336-386) */
+    int32_t periodic;
 } pgfh_config;
 
 typedef struct pgfh_info {
@@ -86,6 +92,19 @@ PGF_API int pgfh_get_enrich_edges(pgfh_problem* p, int32_t* e);
  *  [79] n_coh, [80..81] coh node counts, [82..97] coh nodes (2x8), [98..99] coh laws,
  *  [100] n_enrich, [101..116] enrichment node per slot */
 PGF_API int pgfh_get_cut(pgfh_problem* p, int32_t* rec);
+
+/* shape-sensitivity tables with respect to the radius of the inclusion that
+ * cuts each element (SensitivityShapeParameter::compute_velocity and
+ * Elem::ShapeFunctionPartials, src/SensitivityShapeParameter.cpp:49-182,
+ * src/elem.cpp:141-292): dgrad [n_vgrad] derivative of every volumetric
+ * point's slot gradients (layout of the descriptor's vpt_grad), divv [n_vpt]
+ * velocity divergence, param [n_cut] inclusion of each cut element, vel
+ * [n_all - n_nodes][3] (may be NULL) enrichment-node velocities.  as_reference
+ * selects the reference's expression for the parent-slot term (see
+ * pgf_sens.cpp) instead of the exact derivative.  Not built for cohesive
+ * meshes. */
+PGF_API int pgfh_shape_sensitivity(pgfh_problem* p, int32_t as_reference, double* dgrad, double* divv, int32_t* param,
+                                   double* vel);
 
 /* CSR pattern and element colouring exactly as pgf_create builds them (host
  * integer work, no device needed): sizes first, then the arrays */

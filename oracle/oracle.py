@@ -39,6 +39,7 @@ def lib():
         L.orc_assemble.restype = C.c_double
         L.orc_assemble.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_isv.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_assemble_abs.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
         L.orc_isv_commit.argtypes = [C.c_void_p]
         L.orc_pcg_last.restype = C.c_int
         L.orc_pcg_last.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int]
@@ -49,6 +50,14 @@ def lib():
         L.orc_law.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_void_p]
         L.orc_sample_range.argtypes = [C.c_int64, C.c_int64]
         L.orc_shape.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_periodic_grid.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_set_fext.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_macro_strain_load.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_homogenize.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_pcg_csr_ms_per_iter.restype = C.c_double
+        L.orc_pcg_csr_ms_per_iter.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_int, C.c_int]
         L.orc_edge_root.restype = C.c_double
         L.orc_edge_root.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -67,6 +76,7 @@ class Oracle:
         """sample=(e0, e1) builds and assembles only elements [e0, e1) (timing)."""
         L = lib()
         L.orc_sample_range(*(sample if sample else (-1, -1)))
+        L.orc_periodic_grid(*(prob.grid if getattr(prob, "periodic", False) else (0, 0, 0)))
         self.prob = prob
         i = prob.info
         self.dim = i.dim
@@ -87,6 +97,7 @@ class Oracle:
                               i.n_nodes, i.n_all_nodes, _p(X), i.n_elem, _p(conn), _p(em), len(E), _p(E), _p(nu),
                               laws.shape[0], _p(laws), i.n_cut, _p(cut), len(bcs), _p(bn), _p(bd), _p(bv))
         L.orc_sample_range(-1, -1)
+        L.orc_periodic_grid(0, 0, 0)
         sz = np.zeros(4, np.int64)
         L.orc_sizes(self.h, _p(sz))
         self.n_free, self.n_pres, self.n_isv, self.n_qp = [int(v) for v in sz]
@@ -126,6 +137,36 @@ class Oracle:
         P = np.zeros(self.n_free + self.n_pres)
         ms = lib().orc_assemble(self.h, dt, int(bvariant), _p(vals), _p(P))
         return vals, P, ms
+
+    def assemble_abs(self, dt=1.0):
+        """per K entry (orc_pattern order) the sum of the absolute values of the
+        products that enter it: the scale of its rounding error"""
+        nnz = lib().orc_pattern(self.h, None, None)
+        vals = np.zeros(nnz)
+        lib().orc_assemble_abs(self.h, dt, _p(vals))
+        return vals
+
+    def set_fext(self, f_free):
+        """external load of the Newton residual (free rows)"""
+        lib().orc_set_fext(self.h, _p(np.ascontiguousarray(f_free, np.float64)))
+
+    def macro_strain_load(self, eps_bar):
+        """-int B^T D eps_bar over all rows (BodyLoadMacroStrainLinear)"""
+        e = np.ascontiguousarray(eps_bar, np.float64)
+        f = np.zeros(self.n_free + self.n_pres)
+        lib().orc_macro_strain_load(self.h, _p(e), _p(f))
+        return f
+
+    def homogenize(self, eps_bar, energy=False):
+        """(average stress, average fluctuation strain, volume[, average
+        sigma : eps]) at the stored solution"""
+        S = 3 if self.dim == 2 else 6
+        e = np.ascontiguousarray(eps_bar, np.float64)
+        sig, eps, vol, en = np.zeros(S), np.zeros(S), np.zeros(1), np.zeros(1)
+        lib().orc_homogenize(self.h, _p(e), _p(sig), _p(eps), _p(vol), _p(en))
+        if energy:
+            return sig, eps, float(vol[0]), float(en[0])
+        return sig, eps, float(vol[0])
 
     def isv(self):
         dt, dc = np.zeros(max(self.n_isv, 1)), np.zeros(max(self.n_isv, 1))
@@ -198,3 +239,18 @@ def shape(dim, nn, xi):
     poly = dim == 2 and nn >= 5
     lib().orc_shape(dim, nn, _p(x), _p(N), _p(dN), _p(vtx) if poly else None, _p(rule) if poly else None)
     return (N, dN, vtx, rule) if poly else (N, dN)
+
+
+def set_threads(n):
+    """threads of the oracle's element loop (1 = the serial reference loop)"""
+    lib().orc_set_threads(int(n))
+
+
+def pcg_ms_per_iter(rowptr, rowsplit, colidx, vals, b, iters, threads):
+    """wall ms per Jacobi-PCG iteration on a given CSR K_ff (CPU baseline)"""
+    rp = np.ascontiguousarray(rowptr, np.int64)
+    rs = np.ascontiguousarray(rowsplit, np.int32)
+    ci = np.ascontiguousarray(colidx, np.int32)
+    v = np.ascontiguousarray(vals, np.float64)
+    bb = np.ascontiguousarray(b, np.float64)
+    return lib().orc_pcg_csr_ms_per_iter(len(bb), _p(rp), _p(rs), _p(ci), _p(v), _p(bb), int(iters), int(threads))

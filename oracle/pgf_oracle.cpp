@@ -16,12 +16,12 @@
 //   * Sparsity: Solver::populate_local_neighbors (src/Solver.cpp:723-829) as
 //     std::set per node, split into K_ff/K_fp/K_pf/K_pp.
 //   * Element matrices: Assembler::assemble_elem_linear / assemble_elem_nonlinear
-//     (src/Assembler.cpp:97-184, 283-484) with the dN-variant SmallStrainKFast
+//     (src/Assembler.cpp:94-207, 305-553) with the dN-variant SmallStrainKFast
 //     (src/Utilities.cpp:196-296) or the B-matrix variant (:428-450, dgemm
 //     order) selected by `bvariant`, cohesive KernelCohesive
 //     (src/AssemblerSmallStrainNonlinear.cpp:73-91, src/Utilities.cpp:465-531),
 //     OP-exponential / bilinear laws (src/Material_OPCohesive.cpp:38-319).
-//   * Scatter: Assembler::set_matrix / set_vec (src/Assembler.cpp:811-850)
+//   * Scatter: Assembler::set_matrix / set_vec (src/Assembler.cpp:811-879)
 //     into std::map rows (MatSetValue ADD_VALUES).
 //   * Solve: Jacobi-PCG in FP64 to a tight tolerance (the reference's direct
 //     MUMPS solve, src/Solver.cpp:71-79, has no offline equivalent); Newton
@@ -208,6 +208,9 @@ struct Problem {
     std::vector<Mat> D;                      // per material
     std::vector<Law> laws;
     int64_t n_isv = 0;
+    // periodic sets of the generated grid (Mesh::build_periodic_nodesets);
+    // set[0] is the primary node; empty without periodicity
+    std::vector<std::vector<int>> psets;
     // dofs
     std::vector<std::vector<int>> dofs;      // [n_all][dim]
     int64_t n_free = 0, n_pres = 0;
@@ -467,14 +470,21 @@ Mat small_strain_K(int dim, const Mat& dN, const Mat& D, bool bvar) {
     return K;
 }
 
+// threads of the element loop (orc_set_threads); 1 = the serial reference loop
+int g_threads = 1;
+
 struct Assembled {
     std::vector<std::map<int, double>> rows;  // all dof rows
     Vec P;
 };
 
 // element K_e, P_e (assemble_elem_nonlinear), writes trial delta_max and flags
+// absm: instead of K_e, the bound sum |w| |B|^T |D| |B| (and |w| |N| |N| |R|^T |D_c| |R|
+// for cohesive points) of the absolute values of every product that enters an
+// entry; the rounding error of any summation order of K_e is below a small
+// multiple of the unit roundoff times this sum.
 void element(const Problem& Pb, int64_t e, const Vec& U, double dt, const Vec& dconv, Vec& dtrial,
-             std::vector<int>& flags, bool bvar, Mat& Ke, Vec& Pe) {
+             std::vector<int>& flags, bool bvar, Mat& Ke, Vec& Pe, bool absm = false) {
     const int dim = Pb.dim;
     const auto& nodes = Pb.elem_nodes[e];
     const int nt = (int)nodes.size(), m = nt * dim, s = dim == 2 ? 3 : 6;
@@ -486,7 +496,16 @@ void element(const Problem& Pb, int64_t e, const Vec& U, double dt, const Vec& d
     for (const QP& q : Pb.qps[e]) {
         if (q.coh) continue;
         const Mat& D = Pb.D[q.mat];
-        Mat Kq = small_strain_K(dim, q.grad, D, bvar);
+        Mat Kq;
+        if (absm) {
+            Mat aD = D, aG = q.grad;
+            for (auto& r : aD) for (double& v : r) v = std::fabs(v);
+            for (auto& r : aG) for (double& v : r) v = std::fabs(v);
+            Kq = small_strain_K(dim, aG, aD, false);
+        } else {
+            Kq = small_strain_K(dim, q.grad, D, bvar);
+        }
+        const double wq = absm ? std::fabs(q.w) : q.w;
         Vec eps(s, 0.0), sig(s, 0.0);
         for (int n = 0; n < nt; ++n) {
             const auto& g = q.grad[n];
@@ -506,7 +525,7 @@ void element(const Problem& Pb, int64_t e, const Vec& U, double dt, const Vec& d
             for (int i = 3; i < 6; ++i) sig[i] = D[i][i] * eps[i];
         }
         for (int i = 0; i < m; ++i)
-            for (int j = 0; j < m; ++j) Ke[i][j] += q.w * Kq[i][j];
+            for (int j = 0; j < m; ++j) Ke[i][j] += wq * Kq[i][j];
         for (int n = 0; n < nt; ++n) {
             const auto& g = q.grad[n];
             double f[3];
@@ -548,12 +567,13 @@ void element(const Problem& Pb, int64_t e, const Vec& U, double dt, const Vec& d
             dtrial[q.isv] = dn;
             flags[q.isv] = fl;
             Mat RtD = zeros(dim, dim), RDR = zeros(dim, dim);
+            auto ab = [&](double v) { return absm ? std::fabs(v) : v; };
             for (int i = 0; i < dim; ++i)
                 for (int j = 0; j < dim; ++j)
-                    for (int kk = 0; kk < dim; ++kk) RtD[i][j] += q.R[kk][i] * Dc[kk][j];
+                    for (int kk = 0; kk < dim; ++kk) RtD[i][j] += ab(q.R[kk][i]) * ab(Dc[kk][j]);
             for (int i = 0; i < dim; ++i)
                 for (int j = 0; j < dim; ++j)
-                    for (int kk = 0; kk < dim; ++kk) RDR[i][j] += RtD[i][kk] * q.R[kk][j];
+                    for (int kk = 0; kk < dim; ++kk) RDR[i][j] += RtD[i][kk] * ab(q.R[kk][j]);
             Vec tR(dim, 0.0);
             for (int i = 0; i < dim; ++i)
                 for (int kk = 0; kk < dim; ++kk) tR[i] += q.R[kk][i] * trac[kk];
@@ -562,6 +582,14 @@ void element(const Problem& Pb, int64_t e, const Vec& U, double dt, const Vec& d
                     for (int d1 = 0; d1 < dim; ++d1)
                         for (int d2 = 0; d2 < dim; ++d2) {
                             double v = q.N[n1] * q.N[n2] * RDR[d1][d2];
+                            if (absm) {
+                                const double a = std::fabs(q.w * v);
+                                Kc[n1 * dim + d1][n2 * dim + d2] += a;
+                                Kc[(n1 + nh) * dim + d1][n2 * dim + d2] += a;
+                                Kc[n1 * dim + d1][(n2 + nh) * dim + d2] += a;
+                                Kc[(n1 + nh) * dim + d1][(n2 + nh) * dim + d2] += a;
+                                continue;
+                            }
                             Kc[n1 * dim + d1][n2 * dim + d2] += q.w * v;
                             Kc[(n1 + nh) * dim + d1][n2 * dim + d2] += q.w * -v;
                             Kc[n1 * dim + d1][(n2 + nh) * dim + d2] += q.w * -v;
@@ -585,41 +613,97 @@ void element(const Problem& Pb, int64_t e, const Vec& U, double dt, const Vec& d
     }
 }
 
+// Periodic node sets of a structured grid as Mesh::build_periodic_nodesets
+// (src/Mesh.cpp:603-741) lists them: corners, then (2D) bottom-top and
+// left-right pairs; (3D) x/y/z edge quadruples, then bottom-top, left-right and
+// front-back face pairs.
+std::vector<std::vector<int>> periodic_sets(int dim, int Nx, int Ny, int Nz) {
+    std::vector<std::vector<int>> S;
+    if (dim == 2) {
+        S.push_back({0, Nx, (Nx + 1) * Ny, (Nx + 1) * (Ny + 1) - 1});
+        for (int n = 1; n < Nx; ++n) S.push_back({n, (Nx + 1) * Ny + n});
+        for (int n = 1; n < Ny; ++n) S.push_back({n * (Nx + 1), (n + 1) * (Nx + 1) - 1});
+        return S;
+    }
+    const int top = (Nx + 1) * (Ny + 1) * Nz, lay = (Nx + 1) * (Ny + 1);
+    S.push_back({0, Nx, (Nx + 1) * Ny, lay - 1, top, top + Nx, top + (Nx + 1) * Ny, top + lay - 1});
+    for (int n = 1; n < Nx; ++n) S.push_back({n, (Nx + 1) * Ny + n, n + top, (Nx + 1) * Ny + n + top});
+    for (int n = 1; n < Ny; ++n) S.push_back({n * (Nx + 1), (n + 1) * (Nx + 1) - 1, n * (Nx + 1) + top, (n + 1) * (Nx + 1) - 1 + top});
+    for (int n = 1; n < Nz; ++n) S.push_back({n * lay, n * lay + Nx, n * lay + (Nx + 1) * Ny, n * lay + lay - 1});
+    for (int a = 1; a < Nx; ++a)
+        for (int b = 1; b < Ny; ++b) S.push_back({b * (Nx + 1) + a, b * (Nx + 1) + a + top});
+    for (int a = 1; a < Ny; ++a)
+        for (int b = 1; b < Nz; ++b) S.push_back({a * (Nx + 1) + b * lay, a * (Nx + 1) + b * lay + Nx});
+    for (int a = 1; a < Nx; ++a)
+        for (int b = 1; b < Nz; ++b) S.push_back({a + b * lay, a + b * lay + Ny * (Nx + 1)});
+    return S;
+}
+
+// DofObject::distribute_dofs: free dofs in node order, prescribed after all
+// free ones; with periodic sets only the primary node (set[0]) is numbered and
+// only its Dirichlet conditions count, the other members copy its dofs
+// (set_local_owned_normal_dofs / set_periodic_nodes, src/DofObject.cpp:336-495)
 void number_dofs(Problem& P, const int32_t* bn, const int32_t* bd, const double* bv, int nbc) {
     std::vector<char> isp((size_t)P.n_all * P.dim, 0);
     Vec val((size_t)P.n_all * P.dim, 0.0);
     for (int i = 0; i < nbc; ++i) { isp[(size_t)bn[i] * P.dim + bd[i]] = 1; val[(size_t)bn[i] * P.dim + bd[i]] = bv[i]; }
+    std::vector<char> secondary(P.n_all, 0);
+    for (const auto& set : P.psets)
+        for (size_t k = 1; k < set.size(); ++k) secondary[set[k]] = 1;
     P.n_free = 0;
     P.n_pres = 0;
-    for (char c : isp) (c ? P.n_pres : P.n_free)++;
+    for (int64_t n = 0; n < P.n_all; ++n)
+        if (!secondary[n])
+            for (int d = 0; d < P.dim; ++d) (isp[(size_t)n * P.dim + d] ? P.n_pres : P.n_free)++;
     P.dofs.assign(P.n_all, std::vector<int>(P.dim));
     int64_t f = 0, p = 0;
     P.pres_val.assign(P.n_pres, 0.0);
-    for (int64_t n = 0; n < P.n_all; ++n)
+    for (int64_t n = 0; n < P.n_all; ++n) {
+        if (secondary[n]) continue;
         for (int d = 0; d < P.dim; ++d) {
             if (isp[(size_t)n * P.dim + d]) { P.pres_val[p] = val[(size_t)n * P.dim + d]; P.dofs[n][d] = (int)(P.n_free + p++); }
             else P.dofs[n][d] = (int)f++;
         }
+    }
+    for (const auto& set : P.psets)
+        for (size_t k = 1; k < set.size(); ++k) P.dofs[set[k]] = P.dofs[set[0]];
 }
 
 Assembled assemble(const Problem& Pb, const Vec& U, double dt, const Vec& dconv, Vec& dtrial, std::vector<int>& flags,
-                   bool bvar) {
+                   bool bvar, bool absm = false) {
     Assembled A;
     A.rows.resize(Pb.n_free + Pb.n_pres);
     A.P.assign(Pb.n_free + Pb.n_pres, 0.0);
     dtrial = dconv;
     flags.assign(Pb.n_isv, 0);
-    for (int64_t e = Pb.e_begin; e < Pb.e_end; ++e) {
-        Mat Ke;
-        Vec Pe;
-        element(Pb, e, U, dt, dconv, dtrial, flags, bvar, Ke, Pe);
+    auto scatter = [&](int64_t e, const Mat& Ke, const Vec& Pe) {
         std::vector<int> ed;
         for (int n : Pb.elem_nodes[e])
             for (int d = 0; d < Pb.dim; ++d) ed.push_back(Pb.dofs[n][d]);
         for (size_t i = 0; i < ed.size(); ++i) {
-            for (size_t j = 0; j < ed.size(); ++j) A.rows[ed[i]][ed[j]] += Ke[i][j];
+            for (size_t j = 0; j < ed.size(); ++j) A.rows[ed[i]][ed[j]] += absm ? std::fabs(Ke[i][j]) : Ke[i][j];
             if (Pe[i] != 0.0) A.P[ed[i]] += Pe[i];
         }
+    };
+    if (g_threads <= 1) {
+        for (int64_t e = Pb.e_begin; e < Pb.e_end; ++e) {
+            Mat Ke;
+            Vec Pe;
+            element(Pb, e, U, dt, dconv, dtrial, flags, bvar, Ke, Pe, absm);
+            scatter(e, Ke, Pe);
+        }
+        return A;
+    }
+    // element matrices of a chunk in parallel, scattered in element order by
+    // one thread (the same sums as the serial loop)
+    const int64_t chunk = 8192;
+    std::vector<Mat> Kc(chunk);
+    std::vector<Vec> Pc(chunk);
+    for (int64_t e0 = Pb.e_begin; e0 < Pb.e_end; e0 += chunk) {
+        const int64_t e1 = std::min(Pb.e_end, e0 + chunk);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads)
+        for (int64_t e = e0; e < e1; ++e) element(Pb, e, U, dt, dconv, dtrial, flags, bvar, Kc[e - e0], Pc[e - e0], absm);
+        for (int64_t e = e0; e < e1; ++e) scatter(e, Kc[e - e0], Pc[e - e0]);
     }
     return A;
 }
@@ -680,13 +764,62 @@ struct OrcHandle {
     Vec U;            // [dim][n_all]
     Vec dconv, dtrial;
     std::vector<int> flags;
+    Vec fext;         // [n_free] external load of the Newton residual (zero by default)
 };
+
+// Voigt strain and stress of a volumetric point (SmallStrainFast and
+// computeStress for the isotropic D)
+void point_strain(int dim, const QP& q, const Vec& ue, Vec& eps) {
+    const int nt = (int)q.grad.size();
+    eps.assign(dim == 2 ? 3 : 6, 0.0);
+    for (int n = 0; n < nt; ++n) {
+        const auto& g = q.grad[n];
+        if (dim == 2) {
+            eps[0] += g[0] * ue[n * 2]; eps[1] += g[1] * ue[n * 2 + 1]; eps[2] += g[1] * ue[n * 2] + g[0] * ue[n * 2 + 1];
+        } else {
+            eps[0] += g[0] * ue[n * 3]; eps[1] += g[1] * ue[n * 3 + 1]; eps[2] += g[2] * ue[n * 3 + 2];
+            eps[3] += g[2] * ue[n * 3 + 1] + g[1] * ue[n * 3 + 2];
+            eps[4] += g[2] * ue[n * 3] + g[0] * ue[n * 3 + 2];
+            eps[5] += g[1] * ue[n * 3] + g[0] * ue[n * 3 + 1];
+        }
+    }
+}
+Vec mat_vec(const Mat& D, const Vec& v) {
+    Vec o(D.size(), 0.0);
+    for (size_t i = 0; i < D.size(); ++i)
+        for (size_t j = 0; j < v.size(); ++j) o[i] += D[i][j] * v[j];
+    return o;
+}
+// B^T s for every node of the point (SmallStrainInternalForceFast), into f[nt*dim]
+void point_BtS(int dim, const QP& q, const Vec& s, double w, Vec& f) {
+    const int nt = (int)q.grad.size();
+    for (int n = 0; n < nt; ++n) {
+        const auto& g = q.grad[n];
+        if (dim == 2) {
+            f[n * 2] += w * (g[0] * s[0] + g[1] * s[2]);
+            f[n * 2 + 1] += w * (g[1] * s[1] + g[0] * s[2]);
+        } else {
+            f[n * 3] += w * (g[0] * s[0] + g[2] * s[4] + g[1] * s[5]);
+            f[n * 3 + 1] += w * (g[1] * s[1] + g[2] * s[3] + g[0] * s[5]);
+            f[n * 3 + 2] += w * (g[2] * s[2] + g[1] * s[3] + g[0] * s[4]);
+        }
+    }
+}
 
 }  // namespace
 
 extern "C" {
 
 static int64_t g_e_begin = -1, g_e_end = -1;
+static int g_periodic[3] = {0, 0, 0};
+
+// The next orc_create treats the mesh as the periodic RVE of an Nx x Ny (x Nz)
+// generated grid; 0, 0, 0 switches periodicity off.
+ORC_API void orc_periodic_grid(int Nx, int Ny, int Nz) {
+    g_periodic[0] = Nx;
+    g_periodic[1] = Ny;
+    g_periodic[2] = Nz;
+}
 
 // Restrict the next orc_create to elements [e0, e1) (tables and assembly):
 // used only to time a bounded sample of a large mesh.  -1, -1 restores the
@@ -723,6 +856,7 @@ ORC_API void* orc_create(int dim, int plane_strain, double tet_scale, int64_t n_
     for (int64_t e = 0; e < n_elem; ++e) P.elem_nodes[e] = {conn[e * 4], conn[e * 4 + 1], conn[e * 4 + 2], conn[e * 4 + 3]};
     P.e_begin = g_e_begin < 0 ? 0 : g_e_begin;
     P.e_end = g_e_end < 0 ? n_elem : std::min<int64_t>(g_e_end, n_elem);
+    if (g_periodic[0] > 0) P.psets = periodic_sets(dim, g_periodic[0], g_periodic[1], g_periodic[2]);
     build_tables(P, cut, n_cut);
     number_dofs(P, bc_node, bc_dof, bc_val, n_bc);
     h->U.assign((size_t)dim * n_all, 0.0);
@@ -732,6 +866,56 @@ ORC_API void* orc_create(int dim, int plane_strain, double tet_scale, int64_t n_
 }
 
 ORC_API void orc_destroy(void* h) { delete (OrcHandle*)h; }
+
+// threads used by the element loop of orc_assemble (CPU-baseline timing)
+ORC_API void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+// Jacobi-PCG iterations on a caller-supplied CSR K_ff (rows [0, n), the first
+// rowsplit[r] entries of each row are its free columns): runs exactly `iters`
+// iterations from x = 0 for the right-hand side b with `threads` OpenMP
+// threads and returns the wall time per iteration in ms (CPU-baseline timing
+// of the Krylov loop; the arithmetic is the oracle's pcg()).
+ORC_API double orc_pcg_csr_ms_per_iter(int64_t n, const int64_t* rowptr, const int32_t* rowsplit, const int32_t* col,
+                                       const double* val, const double* b, int iters, int threads) {
+    if (threads < 1) threads = 1;
+    Vec x(n, 0.0), r(b, b + n), z(n), p(n), q(n), dinv(n, 1.0);
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = rowptr[i]; j < rowptr[i] + rowsplit[i]; ++j)
+            if (col[j] == i && val[j] != 0.0) dinv[i] = 1.0 / val[j];
+    double rz = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : rz)
+    for (int64_t i = 0; i < n; ++i) {
+        z[i] = dinv[i] * r[i];
+        p[i] = z[i];
+        rz += r[i] * z[i];
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    for (int it = 0; it < iters; ++it) {
+        double pq = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : pq)
+        for (int64_t i = 0; i < n; ++i) {
+            double s = 0;
+            for (int64_t j = rowptr[i]; j < rowptr[i] + rowsplit[i]; ++j) s += val[j] * p[col[j]];
+            q[i] = s;
+            pq += p[i] * s;
+        }
+        const double a = rz / pq;
+        double rzn = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : rzn)
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] += a * p[i];
+            r[i] -= a * q[i];
+            z[i] = dinv[i] * r[i];
+            rzn += r[i] * z[i];
+        }
+        const double beta = rzn / rz;
+        rz = rzn;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / iters;
+}
 
 ORC_API void orc_sizes(void* hh, int64_t* out) {
     auto* h = (OrcHandle*)hh;
@@ -760,10 +944,14 @@ ORC_API int64_t orc_pattern(void* hh, int64_t* rowptr, int32_t* colidx) {
         for (int a : P.elem_nodes[e])
             for (int b : P.elem_nodes[e])
                 for (int d = 0; d < P.dim; ++d) nb[a].insert(P.dofs[b][d]);
+    // Solver::combine_periodic_contributions (src/Solver.cpp:832-847)
+    for (const auto& set : P.psets)
+        for (size_t k = 1; k < set.size(); ++k) nb[set[0]].insert(nb[set[k]].begin(), nb[set[k]].end());
     const int64_t n = P.n_free + P.n_pres;
-    std::vector<int> rnode(n);
+    std::vector<int> rnode(n, -1);
     for (int64_t a = 0; a < P.n_all; ++a)
-        for (int d = 0; d < P.dim; ++d) rnode[P.dofs[a][d]] = (int)a;
+        for (int d = 0; d < P.dim; ++d)
+            if (rnode[P.dofs[a][d]] < 0) rnode[P.dofs[a][d]] = (int)a;
     int64_t nnz = 0;
     if (rowptr) rowptr[0] = 0;
     for (int64_t r = 0; r < n; ++r) {
@@ -798,6 +986,18 @@ ORC_API double orc_assemble(void* hh, double dt, int bvariant, double* vals, dou
     }
     if (P) std::memcpy(P, h->last.P.data(), sizeof(double) * h->last.P.size());
     return ms;
+}
+
+// the absolute-value bound of every K entry (see element()), in the CSR order of
+// orc_pattern, at the stored solution; the trial state is left unchanged
+ORC_API void orc_assemble_abs(void* hh, double dt, double* vals) {
+    auto* h = (OrcHandle*)hh;
+    Vec dtrial = h->dtrial;
+    std::vector<int> flags = h->flags;
+    Assembled A = assemble(h->P, h->U, dt, h->dconv, dtrial, flags, false, true);
+    int64_t k = 0;
+    for (auto& row : A.rows)
+        for (auto& kv : row) vals[k++] = kv.second;
 }
 
 ORC_API void orc_isv(void* hh, double* dtrial, double* dconv, int32_t* flags) {
@@ -837,7 +1037,10 @@ ORC_API int orc_newton_step(void* hh, double frac, double dt, double rel_tol, do
         h->last = assemble(P, h->U, dt, h->dconv, h->dtrial, h->flags, bvariant != 0);
         r.assign(P.n_free, 0.0);
         double s = 0;
-        for (int64_t i = 0; i < P.n_free; ++i) { r[i] = -h->last.P[i]; s += r[i] * r[i]; }
+        for (int64_t i = 0; i < P.n_free; ++i) {
+            r[i] = (h->fext.empty() ? 0.0 : frac * h->fext[i]) - h->last.P[i];
+            s += r[i] * r[i];
+        }
         return std::sqrt(s);
     };
     Vec r;
@@ -863,6 +1066,77 @@ ORC_API int orc_newton_step(void* hh, double frac, double dt, double rel_tol, do
         info[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return conv ? it : -it - 1;
+}
+
+// external load of the Newton residual r = frac * fext - P_int (free rows)
+ORC_API void orc_set_fext(void* hh, const double* fext) {
+    auto* h = (OrcHandle*)hh;
+    h->fext.assign(fext, fext + h->P.n_free);
+}
+
+// Macro-strain body load of a linear RVE, BodyLoadMacroStrainLinear::BF_Kernel
+// (src/BodyLoadMacroStrainLinear.cpp:10-30): per volumetric point
+// -W J B^T (D eps_bar), assembled over all rows (Assembler::set_vec order)
+ORC_API void orc_macro_strain_load(void* hh, const double* eps_bar, double* f_all) {
+    auto* h = (OrcHandle*)hh;
+    const Problem& P = h->P;
+    const int dim = P.dim, S = dim == 2 ? 3 : 6;
+    Vec eb(eps_bar, eps_bar + S);
+    std::fill(f_all, f_all + P.n_free + P.n_pres, 0.0);
+    for (int64_t e = P.e_begin; e < P.e_end; ++e) {
+        const int nt = (int)P.elem_nodes[e].size();
+        Vec fe((size_t)nt * dim, 0.0);
+        for (const QP& q : P.qps[e]) {
+            if (q.coh) continue;
+            Vec r1 = mat_vec(P.D[q.mat], eb);
+            for (double& v : r1) v *= -1.0 * q.w;
+            point_BtS(dim, q, r1, 1.0, fe);
+        }
+        for (int n = 0; n < nt; ++n)
+            for (int d = 0; d < dim; ++d) f_all[P.dofs[P.elem_nodes[e][n]][d]] += fe[n * dim + d];
+    }
+}
+
+// SubscaleModel::homogenize_stress_strain (src/SubscaleModel.cpp:42-196) at the
+// stored solution (the periodic fluctuation): volume averages of the stress
+// D (eps(u) + eps_bar), of the fluctuation strain eps(u) and of the energy
+// density sigma : (eps(u) + eps_bar) over every volumetric point, and the
+// volume sum W J.
+ORC_API void orc_homogenize(void* hh, const double* eps_bar, double* avg_stress, double* avg_strain, double* volume,
+                            double* avg_energy) {
+    auto* h = (OrcHandle*)hh;
+    const Problem& P = h->P;
+    const int dim = P.dim, S = dim == 2 ? 3 : 6;
+    Vec sig_sum(S, 0.0), eps_sum(S, 0.0);
+    double vol = 0.0, en = 0.0;
+    for (int64_t e = P.e_begin; e < P.e_end; ++e) {
+        const auto& nodes = P.elem_nodes[e];
+        const int nt = (int)nodes.size();
+        Vec ue((size_t)nt * dim);
+        for (int n = 0; n < nt; ++n)
+            for (int d = 0; d < dim; ++d) ue[n * dim + d] = h->U[(size_t)d * P.n_all + nodes[n]];
+        for (const QP& q : P.qps[e]) {
+            if (q.coh) continue;
+            Vec eps;
+            point_strain(dim, q, ue, eps);
+            Vec tot = eps;
+            for (int i = 0; i < S; ++i) tot[i] += eps_bar[i];
+            Vec sig = mat_vec(P.D[q.mat], tot);
+            for (int i = 0; i < S; ++i) {
+                sig_sum[i] += q.w * sig[i];
+                eps_sum[i] += q.w * eps[i];
+                en += q.w * sig[i] * tot[i];
+            }
+            vol += q.w;
+        }
+    }
+    for (int i = 0; i < S; ++i) {
+        avg_stress[i] = sig_sum[i] / vol;
+        avg_strain[i] = eps_sum[i] / vol;
+    }
+    *volume = vol;
+    // volume average of sigma : eps (Voigt, engineering shear)
+    if (avg_energy) *avg_energy = en / vol;
 }
 
 // single evaluation of the traction-separation law: law = [type, sc, dc, df,

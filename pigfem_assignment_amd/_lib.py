@@ -68,6 +68,7 @@ class HostConfig(C.Structure):
         ("n_coh_law", C.c_int32), ("coh_type", i32p), ("coh_params", dp), ("default_coh_law", C.c_int32),
         ("sphere_zlim_as_reference", C.c_int32), ("tet_weight_scale", C.c_double),
         ("n_bc", C.c_int32), ("bc_set", i32p), ("bc_node", i32p), ("bc_dof", i32p), ("bc_val", dp),
+        ("periodic", C.c_int32),
     ]
 
 
@@ -105,8 +106,11 @@ EXPORTS = [
     "pgf_net_create", "pgf_net_destroy", "pgf_net_last_error", "pgf_net_num_params", "pgf_net_get_params",
     "pgf_net_set_params", "pgf_net_forward", "pgf_net_backward", "pgf_net_add_sample", "pgf_net_num_samples",
     "pgf_net_train", "pgf_learn_attach", "pgf_learn_detach", "pgf_learn_commit", "pgf_learn_record", "pgf_learn_last",
-    "pgf_learn_features", "pgf_batch_pcg", "pgf_rhs_prescribed", "pgf_reactions",
+    "pgf_learn_features", "pgf_batch_pcg", "pgf_rhs_prescribed", "pgf_reactions", "pgf_set_solution_dofs",
+    "pgf_volume_average", "pgf_volume_average_noalloc", "pgfh_shape_sensitivity", "pgf_set_shape_sensitivity",
+    "pgf_shape_pseudo_load",
 ]
+PGF_ELIMIT = -7
 
 _lib = None
 
@@ -144,6 +148,8 @@ def lib():
     L.pgf_batch_pcg.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.c_int32, dp, dp, C.c_double, C.c_int32,
                                 C.POINTER(C.c_int32), dp, dp, C.POINTER(C.c_int32)]
     L.pgf_learn_attach.argtypes = [C.c_void_p, C.c_void_p, dp, C.POINTER(C.c_float)]
+    L.pgf_set_solution_dofs.argtypes = [C.c_void_p, dp, dp]
+    L.pgf_volume_average.argtypes = [C.c_void_p, dp, C.c_int32, C.POINTER(C.c_int64), dp]
     L.pgfh_solve_load_steps.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(LoadOpts), STEP_CB, C.c_void_p,
                                         C.POINTER(LoadInfo)]
     for name in ["pgf_destroy", "pgf_csr_info_get", "pgf_get_csr", "pgf_get_colors", "pgf_set_solution",

@@ -15,7 +15,7 @@ class HostProblem:
 
     def __init__(self, kind, N, mins, maxs, inclusions=(), materials=((3.2e9, 0.35),), incl_mat=None,
                  plane_strain=True, cohesive=False, coh_laws=(), incl_coh_law=None, default_coh_law=0,
-                 bcs=(), tet_weight_scale=1.0, sphere_zlim_as_reference=False):
+                 bcs=(), tet_weight_scale=1.0, sphere_zlim_as_reference=False, periodic=False):
         lib = L.lib()
         self._keep = []
         cfg = L.HostConfig()
@@ -57,6 +57,7 @@ class HostProblem:
         cfg.n_bc = len(bcs)
         cfg.bc_set, cfg.bc_node = L.ptr(bset, C.c_int32), L.ptr(bnode, C.c_int32)
         cfg.bc_dof, cfg.bc_val = L.ptr(bdof, C.c_int32), L.ptr(bval, C.c_double)
+        cfg.periodic = int(bool(periodic))
         self._keep += [inc, im, icl, E, nu, ct, cp, bset, bnode, bdof, bval]
         h = C.c_void_p()
         L.check(lib.pgfh_problem_create(C.byref(cfg), C.byref(h)), host=True)
@@ -69,14 +70,21 @@ class HostProblem:
         self.kind = kind
         self.tet_weight_scale = tet_weight_scale
         self.inclusions = inc
+        self.periodic = bool(periodic)
+        self.grid = tuple(int(N[d]) if d < len(N) else 0 for d in range(3))
 
     def bc_list(self):
-        """Dirichlet conditions as (node, dof, full-load value) over regular nodes."""
+        """Dirichlet conditions as (node, dof, full-load value) over regular
+        nodes (for a periodic mesh only the nodes that own their dofs)."""
         dof, pres = self.dofs()
         nf = self.info.n_free
         out = []
+        seen = set()
         for d in range(self.dim):
             for n in np.nonzero(dof[d, : self.info.n_nodes] >= nf)[0]:
+                if (d, int(dof[d, n])) in seen:
+                    continue
+                seen.add((d, int(dof[d, n])))
                 out.append((int(n), d, float(pres[dof[d, n] - nf])))
         return out
 
@@ -192,6 +200,32 @@ class DeviceContext:
         self._chk(self.lib.pgf_get_solution(self.h, L.ptr(u, C.c_double)))
         return u
 
+    def set_solution_dofs(self, x_free, u_pres=None):
+        """nodal field from a dof vector (periodic images take their dof's value)"""
+        x = np.ascontiguousarray(x_free, np.float64)
+        up = None if u_pres is None else np.ascontiguousarray(u_pres, np.float64)
+        self._chk(self.lib.pgf_set_solution_dofs(self.h, L.ptr(x, C.c_double), L.ptr(up, C.c_double)))
+
+    def volume_average(self, eps_bar, seg_elem=None):
+        """per element segment: (average stress [S], average fluctuation strain
+        [S], volume) at the current nodal solution and macro strain eps_bar"""
+        e = np.ascontiguousarray(eps_bar, np.float64)
+        n_elem = self.prob.n_elem if hasattr(self.prob, "n_elem") else self.prob.info.n_elem
+        seg = np.ascontiguousarray([0, n_elem] if seg_elem is None else seg_elem, np.int64)
+        S = len(e)
+        n = len(seg) - 1
+        out = np.zeros((n, 2 * S + 1))
+        self._chk(self.lib.pgf_volume_average(self.h, L.ptr(e, C.c_double), n, seg.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              L.ptr(out, C.c_double)))
+        return out[:, :S], out[:, S:2 * S], out[:, 2 * S]
+
+    def set_load(self, u_pres=None, fext=None):
+        """full-load prescribed values [n_pres] (default: the problem's) and
+        external free-dof force [n_free] (default: none)"""
+        up = self.pres_full if u_pres is None else np.ascontiguousarray(u_pres, np.float64)
+        f = None if fext is None else np.ascontiguousarray(fext, np.float64)
+        self._chk(self.lib.pgf_set_load(self.h, L.ptr(up, C.c_double), L.ptr(f, C.c_double)))
+
     def apply_load_fraction(self, frac):
         self._chk(self.lib.pgf_apply_load_fraction(self.h, frac))
 
@@ -223,6 +257,10 @@ class DeviceContext:
 
     def isv_commit(self):
         self._chk(self.lib.pgf_isv_commit(self.h))
+
+    def isv_rollback(self):
+        """trial delta_max back to the last committed state (pgf_isv_rollback)"""
+        self._chk(self.lib.pgf_isv_rollback(self.h))
 
     def set_isv(self, dmax_conv):
         d = np.ascontiguousarray(dmax_conv, np.float64)

@@ -1,5 +1,16 @@
 """Batches of 2D RVEs solved together on one device context.
 
+Two homogenisation set-ups are provided:
+
+* periodic (the reference's linear subscale model, SubscaleModelLinear,
+  src/SubscaleModelLinear.cpp): a periodic mesh whose upper-face nodes share
+  the dofs of their lower-face images (src/Mesh.cpp:603-741), node 0 fixed
+  (src/SubscaleModel.cpp:232-243), the macro strain applied as the body load
+  -int B^T D eps_bar (src/BodyLoadMacroStrainLinear.cpp:10-30), and the
+  effective stiffness from the volume-averaged stress
+  (src/SubscaleModel.cpp:42-196);
+* affine boundary displacements, described below.
+
 Each RVE is an N x N QUAD4 unit square with seeded circular inclusions, built
 by the host IGFEM set-up exactly like a single problem.  The RVE meshes are
 concatenated into one mesh description (regular nodes of all RVEs, then their
@@ -57,6 +68,14 @@ def rve_problem(N, seed, materials=(configs.MATRIX, configs.INCLUSION), inclusio
                 bcs.append((L.SET_NODE, node, 0, 0.0))
                 bcs.append((L.SET_NODE, node, 1, 0.0))
     return HostProblem("quad4", (N, N), (0, 0), (1, 1), inclusions=incl, materials=materials, bcs=bcs), incl
+
+
+def periodic_rve_problem(N, seed, materials=(configs.MATRIX, configs.INCLUSION), inclusions=None):
+    """Periodic N x N QUAD4 RVE with node 0 (all corners) fixed."""
+    incl = rve_inclusions(N, seed) if inclusions is None else inclusions
+    bcs = [(L.SET_NODE, 0, 0, 0.0), (L.SET_NODE, 0, 1, 0.0)]
+    return HostProblem("quad4", (N, N), (0, 0), (1, 1), inclusions=incl, materials=materials, bcs=bcs,
+                       periodic=True), incl
 
 
 def _arr(ptr, n, dtype):
@@ -131,6 +150,8 @@ class BatchProblem:
             g_off += ng
             pres.append(p.dofs()[1])
         self.n_nodes, self.n_all_nodes, self.n_elem = NR, n_all, e_off
+        self.elem_ptr = np.concatenate([[0], np.cumsum([d.n_elem for d in descs])]).astype(np.int64)
+        self.reg_ptr = R_off.astype(np.int64)
         self.n_free, self.n_pres = NF, NP
         keep = dict(
             coords=np.ascontiguousarray(coords.ravel()), dof=np.ascontiguousarray(dof.ravel()),
@@ -219,3 +240,42 @@ def effective_stiffness(batch, R):
             sxy = 0.5 * (np.sum(Rk[comp == 0] * y[comp == 0]) + np.sum(Rk[comp == 1] * x[comp == 1]))
             Ce[k, :, c] = (sxx, syy, sxy)
     return Ce
+
+
+# Voigt macro strains of the three load cases (engineering shear)
+VOIGT_CASES = [np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0]), np.array([0.0, 0.0, 1.0])]
+
+
+def affine_field(prob, E):
+    """nodal field E x on the regular nodes, zero enrichment amplitudes"""
+    X = prob.coords_full if hasattr(prob, "coords_full") else prob.coords()
+    nr = prob.n_nodes if hasattr(prob, "n_nodes") else prob.info.n_nodes
+    U = np.zeros((2, X.shape[1]))
+    U[:, :nr] = E @ X[:2, :nr]
+    return U
+
+
+def macro_strain_rhs(ctx, prob, cases=CASES):
+    """Right-hand sides of the macro-strain load cases, [n_case][n_free]: the
+    body load -int B^T D eps_bar assembled on the device as minus the internal
+    force of the affine field eps_bar x (its strain is eps_bar at every
+    integration point, cut elements included), periodic images combined."""
+    nf = ctx.csr.n_free
+    B = np.zeros((len(cases), nf))
+    for k, E in enumerate(cases):
+        ctx.set_solution(affine_field(prob, E))
+        ctx.assemble(L.PGF_ASM_FUNC)
+        B[k] = -ctx.internal_force()[:nf]
+    return B
+
+
+def homogenized_stiffness(ctx, prob, X, seg_elem, cases=VOIGT_CASES):
+    """C_eff [n_seg][3][3] (Voigt) from the volume-averaged stress of each load
+    case's fluctuation solution X[k] (free dofs; prescribed dofs zero)."""
+    n_seg = len(seg_elem) - 1
+    C_eff = np.zeros((n_seg, 3, 3))
+    for k, e in enumerate(cases):
+        ctx.set_solution_dofs(X[k])
+        sig, _, _ = ctx.volume_average(e, seg_elem)
+        C_eff[:, :, k] = sig
+    return C_eff

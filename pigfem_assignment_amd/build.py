@@ -13,15 +13,15 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpgfem.so")
 
-HOST_SRC = ["host/pgf_mesh.cpp", "host/pgf_dofs.cpp", "host/pgf_problem.cpp"]
-HIP_SRC = ["kernels/pgf_assemble.hip", "kernels/pgf_device.hip", "kernels/pgf_cnn.hip", "kernels/pgf_batch.hip"]
+HOST_SRC = ["host/pgf_mesh.cpp", "host/pgf_dofs.cpp", "host/pgf_problem.cpp", "host/pgf_sens.cpp"]
+HIP_SRC = ["kernels/pgf_assemble.hip", "kernels/pgf_device.hip", "kernels/pgf_cnn.hip", "kernels/pgf_batch.hip",
+           "kernels/pgf_sens.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 CXXFLAGS = ["-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden", "-I" + os.path.join(ROOT, "include")]
 # packed FP32 instructions are disabled for every device code object
-HIPFLAGS = CXXFLAGS + ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                       "-munsafe-fp-atomics"]
+HIPFLAGS = CXXFLAGS + ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def _run(cmd):

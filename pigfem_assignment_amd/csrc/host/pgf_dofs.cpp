@@ -36,14 +36,22 @@ void DofMap::build(const Mesh& mesh, const std::vector<Dirichlet>& bcs) {
         val[(size_t)b.node * dim + b.dof] = b.value;
     }
     dof.assign((size_t)dim * n_all, -1);
+    // periodic mesh: only the primary node of a periodic set is numbered (and
+    // only its Dirichlet conditions count, as DofObject checks
+    // has_dirichlet_bc for the owned primary node); the other nodes of the set
+    // share its dofs (DofObject::set_periodic_nodes, src/DofObject.cpp:478-495)
+    const std::vector<int32_t>* prim = mesh.periodic() ? &mesh.periodic_primary() : nullptr;
+    auto numbered = [&](int64_t n) { return !prim || n >= mesh.n_nodes() || (*prim)[n] == n; };
     n_free = 0;
     n_pres = 0;
     for (int64_t n = 0; n < n_all; ++n)
-        for (int d = 0; d < dim; ++d)
-            if (is_p[(size_t)n * dim + d]) ++n_pres; else ++n_free;
+        if (numbered(n))
+            for (int d = 0; d < dim; ++d)
+                if (is_p[(size_t)n * dim + d]) ++n_pres; else ++n_free;
     int64_t f = 0, p = 0;
     pres_value.assign(n_pres, 0.0);
-    for (int64_t n = 0; n < n_all; ++n)
+    for (int64_t n = 0; n < n_all; ++n) {
+        if (!numbered(n)) continue;
         for (int d = 0; d < dim; ++d) {
             if (is_p[(size_t)n * dim + d]) {
                 pres_value[p] = val[(size_t)n * dim + d];
@@ -52,6 +60,11 @@ void DofMap::build(const Mesh& mesh, const std::vector<Dirichlet>& bcs) {
                 dof[(size_t)d * n_all + n] = (int32_t)f++;
             }
         }
+    }
+    if (prim)
+        for (int64_t n = 0; n < mesh.n_nodes(); ++n)
+            if ((*prim)[n] != n)
+                for (int d = 0; d < dim; ++d) dof[(size_t)d * n_all + n] = dof[(size_t)d * n_all + (*prim)[n]];
     if (n_free + n_pres > INT32_MAX) throw Error(PGF_ERR_LIMIT, "too many dofs for int32 ids");
 }
 
@@ -93,24 +106,49 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
     const int64_t na = mesh.n_all, ne = mesh.n_elem;
     const int32_t* conn = mesh.conn;
     PhaseTimer tm;
-    // node -> element incidence over every element slot (regular and enrichment)
+    // Nodes that share their dofs (the images of a periodic set) are one node
+    // for the pattern: canon[n] is the first node holding n's dof ids, its rows
+    // take the contributions of every element of every image
+    // (Solver::combine_periodic_contributions, src/Solver.cpp:832-847).  Without
+    // shared dofs canon[n] == n.
+    std::vector<int32_t> canon(na);
+    {
+        std::vector<int32_t> first((size_t)n_free_ + n_pres_, -1);
+        for (int64_t n = 0; n < na; ++n) {
+            const int32_t g = dofarr[n];  // component 0
+            if (g < 0 || g >= n_free_ + n_pres_) throw Error(PGF_ERR_ARG, "dof id out of range");
+            if (first[g] < 0) first[g] = (int32_t)n;
+            canon[n] = first[g];
+        }
+        for (int64_t n = 0; n < na; ++n)
+            for (int d = 1; d < dim; ++d)
+                if (dofarr[(size_t)d * na + n] != dofarr[(size_t)d * na + canon[n]])
+                    throw Error(PGF_ERR_ARG, "nodes sharing a dof must share all their dofs");
+    }
+    shared_dofs = false;
+    for (int64_t n = 0; n < na; ++n) shared_dofs = shared_dofs || canon[n] != n;
+    // node -> element incidence over every element slot (regular and enrichment),
+    // listed under the canonical node, in element order
     std::vector<int64_t> nptr(na + 1, 0);
     for (int64_t e = 0; e < ne; ++e)
-        for (int k = 0; k < 4; ++k) ++nptr[conn[e * 4 + k] + 1];
+        for (int k = 0; k < 4; ++k) ++nptr[canon[conn[e * 4 + k]] + 1];
     for (int64_t c = 0; c < mesh.n_cut; ++c)
-        for (int s = 0; s < mesh.cut_n_enrich[c]; ++s) ++nptr[mesh.n_nodes + mesh.cut_enrich[c * 16 + s] + 1];
+        for (int s = 0; s < mesh.cut_n_enrich[c]; ++s) ++nptr[canon[mesh.n_nodes + mesh.cut_enrich[c * 16 + s]] + 1];
     for (int64_t n = 0; n < na; ++n) nptr[n + 1] += nptr[n];
     std::vector<int32_t> nel(nptr[na]);
     {
         std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
         for (int64_t e = 0; e < ne; ++e)
-            for (int k = 0; k < 4; ++k) nel[fill[conn[e * 4 + k]]++] = (int32_t)e;
+            for (int k = 0; k < 4; ++k) nel[fill[canon[conn[e * 4 + k]]]++] = (int32_t)e;
         for (int64_t c = 0; c < mesh.n_cut; ++c)
             for (int s = 0; s < mesh.cut_n_enrich[c]; ++s)
-                nel[fill[mesh.n_nodes + mesh.cut_enrich[c * 16 + s]]++] = mesh.cut_elem[c];
+                nel[fill[canon[mesh.n_nodes + mesh.cut_enrich[c * 16 + s]]]++] = mesh.cut_elem[c];
+        if (shared_dofs)
+#pragma omp parallel for schedule(dynamic, 4096)
+            for (int64_t n = 0; n < na; ++n) std::sort(nel.begin() + nptr[n], nel.begin() + nptr[n + 1]);
     }
     tm.mark("incidence");
-    // node neighbour lists (sorted all-node ids)
+    // node neighbour lists (sorted canonical node ids)
     std::vector<int64_t> nbptr(na + 1, 0);
     std::vector<std::vector<int32_t>> nbtmp(na);
 #pragma omp parallel for schedule(dynamic, 4096)
@@ -119,7 +157,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
         for (int64_t k = nptr[n]; k < nptr[n + 1]; ++k) {
             int64_t e = nel[k];
             int nt = mesh.n_total(e);
-            for (int s = 0; s < nt; ++s) v.push_back(mesh.node(e, s));
+            for (int s = 0; s < nt; ++s) v.push_back(canon[mesh.node(e, s)]);
         }
         std::sort(v.begin(), v.end());
         v.erase(std::unique(v.begin(), v.end()), v.end());
@@ -160,7 +198,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
         for (int d = 0; d < dim; ++d) {
             int32_t r = dofarr[(size_t)d * na + n];
             if (r < 0 || r >= n_rows) throw Error(PGF_ERR_ARG, "dof id out of range");
-            row_node[r] = n;
+            row_node[r] = canon[n];
         }
     rowptr.assign(n_rows + 1, 0);
     for (int64_t r = 0; r < n_rows; ++r) rowptr[r + 1] = rowptr[r] + rowlen[row_node[r]];
@@ -182,6 +220,8 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
     tm.mark("csr rows");
     // element scatter offsets
     auto lookup = [&](int32_t a, int32_t b, uint8_t& of, uint8_t& op) {
+        a = canon[a];
+        b = canon[b];
         const int32_t* beg = &nb[nbptr[a]];
         const int32_t* end = &nb[nbptr[a + 1]];
         const int32_t* it = std::lower_bound(beg, end, b);
@@ -250,14 +290,18 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
     inc_elem.assign(nel.begin(), nel.end());
     inc_slot.assign(nel.size(), 0);
     inc_off.assign(nel.size(), 0);
-#pragma omp parallel for schedule(dynamic, 4096)
+    bool two_images = false;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(|| : two_images)
     for (int64_t n = 0; n < na; ++n)
         for (int64_t k = nptr[n]; k < nptr[n + 1]; ++k) {
             const int64_t e = nel[k];
             const int nt = mesh.n_total(e);
             int la = -1;
             for (int s = 0; s < nt; ++s)
-                if (mesh.node(e, s) == n) { la = s; break; }
+                if (canon[mesh.node(e, s)] == n) {
+                    if (la >= 0) two_images = true;
+                    if (la < 0) la = s;
+                }
             inc_slot[k] = (uint8_t)la;
             if (mesh.cut_index[e] < 0) {
                 uint64_t w = 0;
@@ -268,6 +312,7 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
                 inc_off[k] = w;
             }
         }
+    if (two_images) throw Error(PGF_ERR_ARG, "an element holds two nodes that share their dofs");
     tm.mark("gather plan");
     // greedy colouring in element order over the regular nodes (elements that
     // share an enrichment node always share that edge's regular nodes)
@@ -276,11 +321,11 @@ void Pattern::build(const MeshView& mesh, const int32_t* dofarr, int64_t n_free_
     n_colors = 0;
     for (int64_t e = 0; e < ne; ++e) {
         uint64_t used = 0;
-        for (int k = 0; k < 4; ++k) used |= mask[conn[e * 4 + k]];
+        for (int k = 0; k < 4; ++k) used |= mask[canon[conn[e * 4 + k]]];
         if (used == ~0ull) throw Error(PGF_ERR_LIMIT, "more than 64 colours needed");
         int c = __builtin_ctzll(~used);
         color[e] = c;
-        for (int k = 0; k < 4; ++k) mask[conn[e * 4 + k]] |= 1ull << c;
+        for (int k = 0; k < 4; ++k) mask[canon[conn[e * 4 + k]]] |= 1ull << c;
         n_colors = std::max(n_colors, c + 1);
     }
     tm.mark("colouring");

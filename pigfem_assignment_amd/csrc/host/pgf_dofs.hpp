@@ -74,6 +74,9 @@ struct Pattern {
     std::vector<int32_t> inc_elem;   // [n_inc]
     std::vector<uint8_t> inc_slot;   // [n_inc]
     std::vector<uint64_t> inc_off;   // [n_inc] uncut elements only
+    // nodes that share dofs (periodic images): only the first node holding a
+    // dof owns its rows and incidence list; the others have empty lists
+    bool shared_dofs = false;
     void build(const MeshView& mesh, const int32_t* dof, int64_t n_free, int64_t n_pres);
 };
 

@@ -244,6 +244,20 @@ std::vector<int32_t> Mesh::nodeset(const std::string& name) const {
     return out;
 }
 
+void Mesh::build_periodic_nodesets() {
+    if (_n_nodes == 0) fail(PGF_ERR_STATE, "generate the mesh before its periodic sets");
+    const int64_t Nx = _N[0], Ny = _N[1], Nz = _N[2];
+    if (Nx < 2 || Ny < 2 || (_dim == 3 && Nz < 2)) fail(PGF_ERR_ARG, "a periodic mesh needs at least 2 divisions per side");
+    _periodic_primary.resize(_n_nodes);
+    int64_t id = 0;
+    for (int64_t z = 0; z <= Nz; ++z)
+        for (int64_t y = 0; y <= Ny; ++y)
+            for (int64_t x = 0; x <= Nx; ++x, ++id) {
+                const int64_t px = x % Nx, py = y % Ny, pz = _dim == 3 ? z % Nz : 0;
+                _periodic_primary[id] = (int32_t)(px + py * (Nx + 1) + pz * (Nx + 1) * (Ny + 1));
+            }
+}
+
 // ---------------------------------------------------------------------------
 // Node detection (src/Mesh.cpp:2838-2891, src/InclusionCircle.cpp:150-163,
 // src/InclusionSphere.cpp:163-178): inclusions scanned last-to-first, a node

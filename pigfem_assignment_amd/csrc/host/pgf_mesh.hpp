@@ -117,6 +117,17 @@ class Mesh {
     // nodesets of the generated mesh (upper-case names as in the reference)
     std::vector<int32_t> nodeset(const std::string& name) const;
 
+    // ---- periodic RVE (src/Mesh.cpp:603-741) ----
+    // Periodic images of a generated grid: every node with a coordinate index
+    // at the upper bound N is an image of the node with that index 0 (corners,
+    // edges and faces alike), so each periodic set's primary node -- the
+    // lowest id of the set, the reference's set[0] -- is the node whose grid
+    // indices are (x mod Nx, y mod Ny, z mod Nz).
+    void build_periodic_nodesets();
+    bool periodic() const { return !_periodic_primary.empty(); }
+    // primary node of every regular node's periodic set (itself if none)
+    const std::vector<int32_t>& periodic_primary() const { return _periodic_primary; }
+
     // element-local node slot -> all-node id (regular i -> i, enrichment k -> n_nodes + k)
     int32_t elem_node(int64_t e, int slot) const;
     int elem_n_total(int64_t e) const;
@@ -159,6 +170,7 @@ class Mesh {
     std::vector<CutElem> _cut;
     std::vector<int32_t> _cut_index;   // per element: index into _cut or -1
     std::vector<std::array<int32_t, 3>> _enrich_edge;
+    std::vector<int32_t> _periodic_primary;
     MeshOptions _opt;
     bool _nodes_detected = false, _elements_detected = false, _enriched = false;
 };

@@ -18,6 +18,7 @@
 #include "../../../include/pgfem_host.h"
 #include "pgf_dofs.hpp"
 #include "pgf_mesh.hpp"
+#include "pgf_sens.hpp"
 
 namespace {
 std::string g_host_error;
@@ -64,6 +65,7 @@ PGF_API int pgfh_problem_create(const pgfh_config* cfg, pgfh_problem** out) {
         if (opt.cohesive && cfg->n_coh_law <= 0) throw pgf::Error(PGF_EARG, "cohesive mesh without a cohesive law");
         p->mesh.set_options(opt);
         p->mesh.generate_mesh(cfg->elem_kind == PGF_TET4 ? pgf::TET4 : pgf::QUAD4, cfg->mins, cfg->maxs, cfg->N);
+        if (cfg->periodic) p->mesh.build_periodic_nodesets();
         for (int i = 0; i < cfg->n_incl; ++i) {
             pgf::Inclusion inc;
             inc.type = (int)cfg->incl[i * 5 + 0];
@@ -242,6 +244,24 @@ PGF_API int pgfh_get_cut(pgfh_problem* p, int32_t* rec) {
         for (int i = 0; i < 16; ++i) r[101 + i] = ce.enrich[i];
     }
     return 0;
+}
+
+PGF_API int pgfh_shape_sensitivity(pgfh_problem* p, int32_t as_reference, double* dgrad, double* divv, int32_t* param,
+                                   double* vel) {
+    if (!p || !dgrad || !divv || !param) return PGF_EARG;
+    try {
+        if (p->mesh.is_cohesive()) throw pgf::Error(PGF_EUNSUPPORTED, "shape sensitivities of cohesive meshes are not built");
+        pgf::ShapeSensTables t;
+        pgf::build_shape_sensitivity(p->mesh, as_reference != 0, t);
+        std::memcpy(dgrad, t.dgrad.data(), sizeof(double) * t.dgrad.size());
+        std::memcpy(divv, t.divv.data(), sizeof(double) * t.divv.size());
+        std::memcpy(param, t.param.data(), sizeof(int32_t) * t.param.size());
+        if (vel) std::memcpy(vel, t.vel.data(), sizeof(double) * t.vel.size());
+        return 0;
+    } catch (const pgf::Error& e) {
+        g_host_error = e.what();
+        return e.code;
+    }
 }
 
 PGF_API int pgfh_pattern_info(pgfh_problem* p, int64_t* n_rows, int64_t* nnz, int32_t* n_colors) {

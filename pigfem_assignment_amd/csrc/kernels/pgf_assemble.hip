@@ -8,7 +8,7 @@
 // (SmallStrainInternalForceFast, :364-408), cohesive opening / stiffness /
 // force (cohesiveDeltaFast, cohesiveKFast, cohesiveInternalForceFast,
 // :465-531).  The scatter is Assembler::set_matrix / set_vec
-// (src/Assembler.cpp:811-850).
+// (src/Assembler.cpp:811-879).
 //
 // Two launch schemes, both free of atomics:
 //  * gather (default): one thread per node owns that node's CSR rows, zeroes
@@ -556,7 +556,9 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     const int t = threadIdx.x;
     const int lu = t / SL, slot = t % SL;  // local unit, slot
     const int64_t n = NR == 1 ? n0 + lu / DIM : n0 + lu;
-    const bool active = n < n1;
+    // a periodic image owns no rows (its incidence list is empty; the node
+    // holding its dofs first gathers its elements)
+    const bool active = n < n1 && !(a.image && a.image[n]);
     constexpr int S = DIM == 2 ? 3 : 6;
     constexpr int NQ = DIM == 2 ? 4 : 1;
     int da[NR], r[NR];
@@ -689,6 +691,106 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Homogenised response of an RVE (SubscaleModel::homogenize_stress_strain,
+// src/SubscaleModel.cpp:42-196): over every volumetric point of the elements
+// of a segment, sum w * sigma with sigma = D (eps(u) + eps_bar), w * eps(u)
+// and w.  Segment k (elements [seg[k], seg[k+1])) is split evenly over
+// VOL_BLOCKS blocks; each block writes one partial per quantity, and
+// k_vol_finish adds a segment's partials in block order (deterministic).
+constexpr int VOL_BLOCKS = 64;
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_vol_partials(AsmArgs a, const double* __restrict__ eps_bar,
+                                                      const int64_t* __restrict__ seg, double* __restrict__ part) {
+    constexpr int S = DIM == 2 ? 3 : 6, NQ = DIM == 2 ? 4 : 1, W = 2 * S + 1;
+    const int k = blockIdx.x / VOL_BLOCKS, b = blockIdx.x % VOL_BLOCKS;
+    const int64_t e0 = seg[k], len = seg[k + 1] - e0;
+    const int64_t c0 = e0 + len * b / VOL_BLOCKS, c1 = e0 + len * (b + 1) / VOL_BLOCKS;
+    double acc[W];
+    for (int i = 0; i < W; ++i) acc[i] = 0.0;
+    double eb[S];
+    for (int i = 0; i < S; ++i) eb[i] = eps_bar[i];
+    auto add_point = [&](const Iso& D, const double* eps, double w) {
+        double tot[S], sig[S];
+        for (int i = 0; i < S; ++i) tot[i] = eps[i] + eb[i];
+        stress_of<DIM>(D, tot, sig);
+        for (int i = 0; i < S; ++i) {
+            acc[i] += w * sig[i];
+            acc[S + i] += w * eps[i];
+        }
+        acc[2 * S] += w;
+    };
+    for (int64_t e = c0 + threadIdx.x; e < c1; e += blockDim.x) {
+        const int c = a.cut_index[e];
+        if (c < 0) {
+            double X[4][DIM], u[4 * DIM];
+            for (int q = 0; q < 4; ++q) {
+                const int nd = a.el_conn[(int64_t)q * a.n_elem + e];
+                for (int i = 0; i < DIM; ++i) {
+                    X[q][i] = a.X[i * a.n_all + nd];
+                    u[q * DIM + i] = a.U[i * a.n_all + nd];
+                }
+            }
+            const Iso D = iso_of<DIM>(a.Dmat + a.el_mat[e] * 36);
+            for (int q = 0; q < NQ; ++q) {
+                double g[4 * DIM], w, eps[S];
+                qp_geometry<DIM>(X, a.tet_scale, q, g, w);
+                strain_of<DIM>(4, g, u, eps);
+                add_point(D, eps, w);
+            }
+        } else {
+            const int nt = a.ce_nt[c];
+            double u[MAX_NT * DIM];
+            for (int s = 0; s < nt; ++s) {
+                const int nd = a.ce_nodes[c * 16 + s];
+                for (int i = 0; i < DIM; ++i) u[s * DIM + i] = a.U[i * a.n_all + nd];
+            }
+            for (int64_t pt = a.vpt_ptr[c]; pt < a.vpt_ptr[c + 1]; ++pt) {
+                double eps[S];
+                strain_of<DIM>(nt, a.vpt_grad + a.vpt_grad_off[pt], u, eps);
+                add_point(iso_of<DIM>(a.Dmat + a.vpt_mat[pt] * 36), eps, a.vpt_w[pt]);
+            }
+        }
+    }
+    // block tree sum in a fixed order
+    __shared__ double red[256];
+    for (int i = 0; i < W; ++i) {
+        red[threadIdx.x] = acc[i];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * W + i] = red[0];
+        __syncthreads();
+    }
+}
+
+// out[k] = (averaged stress [S], averaged fluctuation strain [S], volume)
+__global__ void k_vol_finish(int n_seg, int S, const double* __restrict__ part, double* __restrict__ out) {
+    const int W = 2 * S + 1;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_seg; k += gridDim.x * blockDim.x) {
+        double sum[13];
+        for (int i = 0; i < W; ++i) sum[i] = 0.0;
+        for (int b = 0; b < VOL_BLOCKS; ++b)
+            for (int i = 0; i < W; ++i) sum[i] += part[((int64_t)k * VOL_BLOCKS + b) * W + i];
+        const double v = sum[2 * S];
+        for (int i = 0; i < 2 * S; ++i) out[(int64_t)k * W + i] = v != 0.0 ? sum[i] / v : 0.0;
+        out[(int64_t)k * W + 2 * S] = v;
+    }
+}
+
+// nodal field from a dof vector: every node slot takes its dof's value
+// (free part x, prescribed part up), periodic images included
+__global__ void k_set_from_dofs(int64_t n_slots, const int32_t* __restrict__ dof, int64_t n_free,
+                                const double* __restrict__ x, const double* __restrict__ up, double* __restrict__ U) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t g = dof[i];
+        U[i] = g < n_free ? x[g] : (up ? up[g - n_free] : 0.0);
+    }
+}
+
 }  // namespace pgfk
 
 // ---------------------------------------------------------------------------
@@ -757,6 +859,22 @@ static void launch_dim(Ctx& c, const pgfk::AsmArgs& a, bool jac, bool func, bool
     else if (jac && !fp32) launch_gather<DIM, true, false, double>(c, a);
     else if (jac && fp32) launch_gather<DIM, true, false, float>(c, a);
     if (func && (fp32 || !jac)) launch_gather<DIM, false, true, double>(c, a);
+}
+
+void enqueue_volume_average(Ctx& c, const double* d_eps_bar, int n_seg, const int64_t* d_seg, double* d_part,
+                            double* d_out) {
+    const pgfk::AsmArgs a = c.asm_args();
+    const dim3 grid((unsigned)(n_seg * pgfk::VOL_BLOCKS)), blk(256);
+    if (c.dim == 2) hipLaunchKernelGGL((pgfk::k_vol_partials<2>), grid, blk, 0, c.stream, a, d_eps_bar, d_seg, d_part);
+    else hipLaunchKernelGGL((pgfk::k_vol_partials<3>), grid, blk, 0, c.stream, a, d_eps_bar, d_seg, d_part);
+    hipLaunchKernelGGL(pgfk::k_vol_finish, dim3((unsigned)((n_seg + 63) / 64)), dim3(64), 0, c.stream, n_seg,
+                       c.dim == 2 ? 3 : 6, d_part, d_out);
+}
+
+void enqueue_set_from_dofs(Ctx& c, const double* d_x, const double* d_up) {
+    const int64_t n = (int64_t)c.dim * c.n_all;
+    const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(pgfk::k_set_from_dofs, dim3(nb), dim3(256), 0, c.stream, n, c.dof, c.n_free, d_x, d_up, c.U);
 }
 
 void enqueue_assembly(Ctx& c, double dt, uint32_t flags) {

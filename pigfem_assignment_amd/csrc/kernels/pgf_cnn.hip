@@ -1132,6 +1132,12 @@ extern "C" {
 PGF_API int pgf_learn_attach(pgf_ctx* h, pgf_net* net, const double* box, const float* indicator) {
     if (!h || !box) return PGF_EARG;
     pgfd::Ctx* c = pgfd::ctx_of(h);
+    // the voxel features and the prolongation map one node per free dof;
+    // periodic images share dofs at different positions
+    if (c->image) {
+        c->err = "the learned guess does not support periodic meshes";
+        return PGF_EUNSUPPORTED;
+    }
     return cguard(c, [&]() -> int {
         if (c->learn) pgfd::learn_free(c->learn);
         c->learn = nullptr;

@@ -5,7 +5,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <sstream>
+#include <string>
 
 #include "../../../include/pgfem.h"
 #include "../host/pgf_dofs.hpp"
@@ -21,6 +23,31 @@ namespace pgfd {
     } while (0)
 
 static std::string g_last_error;
+
+// Device scratch of the allocation-free entry point (pgf_volume_average_noalloc):
+// staging of eps_bar and the segment list, the per-segment results and the
+// per-block partial sums, sized for PGF_NOALLOC_MAX_SEG segments.  Every GPU
+// holds its own copy; host code uses the address hipGetSymbolAddress returns
+// for the current device, never the array's name.
+constexpr size_t NOALLOC_DOUBLES =
+    8 + (PGF_NOALLOC_MAX_SEG + 1) + (size_t)PGF_NOALLOC_MAX_SEG * 13 + (size_t)PGF_NOALLOC_MAX_SEG * 64 * 13;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[64] = {nullptr};
+
+double* noalloc_scratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) throw pgf::Error(PGF_EDEVICE, "no current device");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw pgf::Error(PGF_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
 
 template <typename T>
 static T* dalloc(int64_t n) {
@@ -60,7 +87,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.cpt_law = cpt_law; a.laws = laws; a.ce_off = ce_off; a.ce_off_ptr = ce_off_ptr; a.ce_rel = ce_rel; a.ce_rel_ptr = ce_rel_ptr;
     a.dmax_conv = dmax_conv; a.dmax_trial = dmax_trial; a.flags = flags;
     a.rowptr = rowptr; a.K = K; a.K32 = K32; a.P = P;
-    a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index;
+    a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index; a.image = image;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_rel = inc_rel;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
@@ -70,10 +97,11 @@ pgfk::AsmArgs Ctx::asm_args() const {
 Ctx::~Ctx() {
     void* ptrs[] = {X, dof, U, ue_conn, ue_mat, ue_off, Dmat, ce_list, ce_nodes, ce_nt, vpt_ptr, vpt_grad_off, cpt_ptr,
                     ce_off_ptr, vpt_w, vpt_grad, cpt_w, cpt_N, cpt_R, vpt_mat, cpt_law, cpt_nodes, cpt_nh, ce_off, laws,
-                    dmax_conv, dmax_trial, flags, free_slot, pres_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
+                    dmax_conv, dmax_trial, flags, free_slot, rowptr, rowsplit, colidx, K, K32, P, Fext_full,
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
-                    inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row, ce_rel, ce_rel_ptr};
+                    inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row, ce_rel, ce_rel_ptr, image,
+                    sens_dgrad, sens_divv, sens_pe, sens_ds, sens_param};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -177,7 +205,9 @@ void build_gather_blocks(Ctx& c, int slots, bool node_rows) {
         int64_t n1 = n0, rmin = INT64_MAX, rmax = -1, cnt = 0, inc_max = 0;
         while (n1 < na && n1 - n0 < per_blk) {
             int64_t mn = rmin, mx = rmax, ct = cnt;
-            for (int dd = 0; dd < c.dim; ++dd) {
+            // a periodic image owns no rows: the node holding its dofs first does
+            const bool owns = c.h_image.empty() || !c.h_image[n1];
+            for (int dd = 0; dd < c.dim && owns; ++dd) {
                 const int64_t g = c.h_dof[(size_t)dd * na + n1];
                 if (g < c.n_free) { mn = std::min(mn, g); mx = std::max(mx, g); ++ct; }
             }
@@ -403,9 +433,14 @@ __global__ void k_diag_inv(int64_t n, const int32_t* rowptr, const int32_t* rows
 }
 
 // ---- Newton helpers ----
-__global__ void k_apply_frac(int64_t n, const int64_t* slot, const double* upf, double frac, double* U) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        U[slot[i]] = frac * upf[i];
+// prescribed values into the nodal field: every node slot (d * n_all + node)
+// whose dof is prescribed, so all nodes sharing a dof (periodic images) get it
+__global__ void k_apply_frac(int64_t n_slots, const int32_t* dof, int64_t n_free, const double* upf, double frac,
+                             double* U) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t g = dof[i];
+        if (g >= n_free) U[i] = frac * upf[g - n_free];
+    }
 }
 // r = frac*Fext - P_int (free rows) and ||r||^2 into scal[slot]
 __global__ __launch_bounds__(256) void k_residual(int64_t n, const double* Fext, double frac, const double* P,
@@ -422,10 +457,13 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, const double* Fext,
         *counter = 0;
     }
 }
+// U += dx over every node slot with a free dof (periodic images included)
 template <typename T>
-__global__ void k_update_U(int64_t n, const int64_t* slot, const T* dx, double* U) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        U[slot[i]] += (double)dx[i];
+__global__ void k_update_U(int64_t n_slots, const int32_t* dof, int64_t n_free, const T* dx, double* U) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t g = dof[i];
+        if (g < n_free) U[i] += (double)dx[g];
+    }
 }
 __global__ void k_copy(int64_t n, const double* a, double* b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
@@ -702,13 +740,15 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
                     if (d->cpt_nodes[q * 8 + s] >= ce_nt[k]) throw pgf::Error(PGF_EARG, "cohesive node slot out of range");
             }
         c->n_cpt = n_cpt;
-        // ---- free / prescribed slots ----
-        std::vector<int64_t> fslot(std::max<int64_t>(d->n_free, 1)), pslot(std::max<int64_t>(d->n_pres, 1));
+        c->n_vpt = n_vpt;
+        c->n_vgrad = n_grad;
+        c->h_cut_elem.assign(d->cut_elem, d->cut_elem + d->n_cut);
+        // ---- free dof -> nodal slot of the first node holding it ----
+        std::vector<int64_t> fslot(std::max<int64_t>(d->n_free, 1), -1);
         for (int dd = 0; dd < d->dim; ++dd)
             for (int64_t n = 0; n < d->n_all_nodes; ++n) {
                 int32_t g = d->dof[(size_t)dd * d->n_all_nodes + n];
-                if (g < d->n_free) fslot[g] = (int64_t)dd * d->n_all_nodes + n;
-                else pslot[g - d->n_free] = (int64_t)dd * d->n_all_nodes + n;
+                if (g < d->n_free && fslot[g] < 0) fslot[g] = (int64_t)dd * d->n_all_nodes + n;
             }
         // ---- upload ----
         std::vector<double> Xh(3 * d->n_all_nodes);
@@ -819,6 +859,17 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->ke = pgfd::dalloc<double>(kp[d->n_cut]);
             c->pe = pgfd::dalloc<double>(pp[d->n_cut]);
             c->h_dof.assign(d->dof, d->dof + (size_t)d->dim * d->n_all_nodes);
+            if (pat.shared_dofs) {
+                // periodic images: nodes whose dofs an earlier node holds
+                std::vector<int32_t> first((size_t)c->n_rows, -1);
+                c->h_image.assign(d->n_all_nodes, 0);
+                for (int64_t n = 0; n < d->n_all_nodes; ++n) {
+                    const int32_t g = d->dof[n];
+                    if (first[g] < 0) first[g] = (int32_t)n;
+                    else c->h_image[n] = 1;
+                }
+                c->image = pgfd::dupload(c->h_image.data(), d->n_all_nodes);
+            }
             c->h_inc_ptr = pat.inc_ptr;
             pgfd::build_gather_blocks(*c, pgfd::default_gather_slots(d->dim), false);
         }
@@ -829,7 +880,6 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         HIPCHK(hipMemset(c->dmax_trial, 0, sizeof(double) * std::max<int64_t>(n_cpt, 1)));
         HIPCHK(hipMemset(c->flags, 0, std::max<int64_t>(n_cpt, 1)));
         c->free_slot = pgfd::dupload(fslot.data(), (int64_t)fslot.size());
-        c->pres_slot = pgfd::dupload(pslot.data(), (int64_t)pslot.size());
         std::vector<int32_t> rp32(c->n_rows + 1);
         for (int64_t r = 0; r <= c->n_rows; ++r) rp32[r] = (int32_t)pat.rowptr[r];
         c->rowptr = pgfd::dupload(rp32.data(), c->n_rows + 1);
@@ -916,6 +966,68 @@ PGF_API int pgf_set_solution(pgf_ctx* c, const double* u) {
     });
 }
 
+PGF_API int pgf_set_solution_dofs(pgf_ctx* c, const double* x_free, const double* u_pres) {
+    if (!c || !x_free) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        double* dx = pgfd::dupload(x_free, std::max<int64_t>(c->n_free, 1));
+        double* dp = u_pres ? pgfd::dupload(u_pres, std::max<int64_t>(c->n_pres, 1)) : nullptr;
+        pgfd::enqueue_set_from_dofs(*c, dx, dp);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(dx));
+        if (dp) HIPCHK(hipFree(dp));
+        return 0;
+    });
+}
+
+PGF_API int pgf_volume_average(pgf_ctx* c, const double* eps_bar, int32_t n_seg, const int64_t* seg_elem, double* out) {
+    if (!c || !eps_bar || n_seg <= 0 || !seg_elem || !out) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        for (int32_t k = 0; k < n_seg; ++k)
+            if (seg_elem[k] < 0 || seg_elem[k] > seg_elem[k + 1] || seg_elem[k + 1] > c->n_elem)
+                throw pgf::Error(PGF_EARG, "element segments must be ordered ranges inside [0, n_elem]");
+        const int S = c->dim == 2 ? 3 : 6, W = 2 * S + 1;
+        double* de = pgfd::dupload(eps_bar, S);
+        int64_t* ds = pgfd::dupload(seg_elem, (int64_t)n_seg + 1);
+        double* dpart = pgfd::dalloc<double>((int64_t)n_seg * 64 * W);
+        double* dout = pgfd::dalloc<double>((int64_t)n_seg * W);
+        pgfd::enqueue_volume_average(*c, de, n_seg, ds, dpart, dout);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * n_seg * W, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (void* p : {(void*)de, (void*)ds, (void*)dpart, (void*)dout}) HIPCHK(hipFree(p));
+        return 0;
+    });
+}
+
+PGF_API int pgf_volume_average_noalloc(pgf_ctx* c, const double* eps_bar, int32_t n_seg, const int64_t* seg_elem,
+                                      double* out) {
+    if (!c || !eps_bar || n_seg <= 0 || !seg_elem || !out) return PGF_EARG;
+    if (n_seg > PGF_NOALLOC_MAX_SEG) {
+        c->err = "pgf_volume_average_noalloc: at most " + std::to_string(PGF_NOALLOC_MAX_SEG) + " segments";
+        return PGF_ELIMIT;
+    }
+    return guard(c, [&]() -> int {
+        for (int32_t k = 0; k < n_seg; ++k)
+            if (seg_elem[k] < 0 || seg_elem[k] > seg_elem[k + 1] || seg_elem[k + 1] > c->n_elem)
+                throw pgf::Error(PGF_EARG, "element segments must be ordered ranges inside [0, n_elem]");
+        double* base = pgfd::noalloc_scratch();
+        const int S = c->dim == 2 ? 3 : 6, W = 2 * S + 1;
+        // scratch layout: eps_bar [8] | segments [MAX_SEG + 1] int64 | out [MAX_SEG][13] | partials
+        double* de = base;
+        int64_t* ds = (int64_t*)(base + 8);
+        double* dout = base + 8 + (PGF_NOALLOC_MAX_SEG + 1);
+        double* dpart = dout + (size_t)PGF_NOALLOC_MAX_SEG * 13;
+        HIPCHK(hipMemcpyAsync(de, eps_bar, sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(ds, seg_elem, sizeof(int64_t) * (n_seg + 1), hipMemcpyHostToDevice, c->stream));
+        pgfd::enqueue_volume_average(*c, de, n_seg, ds, dpart, dout);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * n_seg * W, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
 PGF_API int pgf_get_solution(pgf_ctx* c, double* u) {
     if (!c || !u) return PGF_EARG;
     return guard(c, [&]() -> int {
@@ -941,8 +1053,8 @@ PGF_API int pgf_apply_load_fraction(pgf_ctx* c, double frac) {
     if (!c) return PGF_EARG;
     return guard(c, [&]() -> int {
         if (c->n_pres)
-            hipLaunchKernelGGL(pgfd::k_apply_frac, dim3(pgfd::grid_for(c->n_pres)), dim3(256), 0, c->stream, c->n_pres,
-                               c->pres_slot, c->upres_full, frac, c->U);
+            hipLaunchKernelGGL(pgfd::k_apply_frac, dim3(pgfd::grid_for(c->dim * c->n_all)), dim3(256), 0, c->stream,
+                               (int64_t)c->dim * c->n_all, c->dof, c->n_free, c->upres_full, frac, c->U);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         return 0;
@@ -1084,8 +1196,8 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
         const int64_t nf = c->n_free;
         double ms_asm = 0.0, ms_sol = 0.0;
         if (c->n_pres)
-            hipLaunchKernelGGL(pgfd::k_apply_frac, dim3(pgfd::grid_for(c->n_pres)), dim3(256), 0, c->stream, c->n_pres,
-                               c->pres_slot, c->upres_full, frac, c->U);
+            hipLaunchKernelGGL(pgfd::k_apply_frac, dim3(pgfd::grid_for(c->dim * c->n_all)), dim3(256), 0, c->stream,
+                               (int64_t)c->dim * c->n_all, c->dof, c->n_free, c->upres_full, frac, c->U);
         auto residual = [&](double& rn) {
             HIPCHK(hipEventRecord(a0, c->stream));
             pgfd::enqueue_assembly(*c, o->dt, PGF_ASM_FUNC);
@@ -1154,8 +1266,8 @@ PGF_API int pgf_newton_step(pgf_ctx* c, double frac, const pgf_newton_opts* o, p
             } else
                 pgfd::run_pcg<double, double>(*c, c->K, c->r, true, c->x, c->r, c->z, c->p0, c->p1, c->q, c->dinv, g,
                                               lin_tol, o->lin_maxit, &si);
-            hipLaunchKernelGGL((pgfd::k_update_U<double>), dim3(pgfd::grid_for(nf)), dim3(256), 0, c->stream, nf,
-                               c->free_slot, c->x, c->U);
+            hipLaunchKernelGGL((pgfd::k_update_U<double>), dim3(pgfd::grid_for(c->dim * c->n_all)), dim3(256), 0,
+                               c->stream, (int64_t)c->dim * c->n_all, c->dof, nf, c->x, c->U);
             ms_sol += si.ms;
             pcg_its += si.iterations;
             ++it;

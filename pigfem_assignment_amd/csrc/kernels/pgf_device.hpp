@@ -55,10 +55,18 @@ struct Ctx {
     int64_t n_gblk = 0, seg_max = 0;
     int gslots = 1;                 // gather pass: threads per row (or per node with gnode)
     bool gnode = false;             // gather pass: one thread serves all rows of a node
+    int64_t n_vpt = 0, n_vgrad = 0;  // volumetric points of the cut elements and their gradient entries
+    std::vector<int32_t> h_cut_elem; // [n_cut] parent element of each cut element
+    // shape-sensitivity tables and scratch (pgf_sens.hip)
+    double *sens_dgrad = nullptr, *sens_divv = nullptr, *sens_pe = nullptr, *sens_ds = nullptr;
+    int32_t* sens_param = nullptr;
+    std::vector<int32_t> h_sens_param;
+    uint8_t* image = nullptr;        // [n_all] periodic images (null without shared dofs)
+    std::vector<uint8_t> h_image;    // host copy (empty without shared dofs)
     std::vector<int32_t> h_dof;      // [dim][n_all] (gather partition)
     std::vector<int64_t> h_inc_ptr;  // [n_all+1] node -> element incidence
-    // free / prescribed dof -> nodal field slot (d*n_all + node)
-    int64_t *free_slot = nullptr, *pres_slot = nullptr;
+    // free dof -> nodal field slot (d*n_all + node) of the first node holding it
+    int64_t* free_slot = nullptr;
     // matrix
     int32_t *rowptr = nullptr, *rowsplit = nullptr, *colidx = nullptr;
     int32_t* grp_row = nullptr;  // [n_grp+1] first free row of each SpMV node group
@@ -87,6 +95,13 @@ struct Ctx {
 template <typename T>
 void fill_async(Ctx& c, T* p, int64_t n, T v);
 void enqueue_assembly(Ctx& c, double dt, uint32_t flags);
+// homogenised response per element segment (partials [n_seg * 64][2S+1], out [n_seg][2S+1])
+void enqueue_volume_average(Ctx& c, const double* d_eps_bar, int n_seg, const int64_t* d_seg, double* d_part,
+                            double* d_out);
+// the current device's scratch of the allocation-free entry point
+double* noalloc_scratch();
+// nodal field U from a dof vector (free part x, prescribed part up or zero)
+void enqueue_set_from_dofs(Ctx& c, const double* d_x, const double* d_up);
 int default_gather_slots(int dim);
 void build_gather_blocks(Ctx& c, int slots, bool node_rows);
 

@@ -87,6 +87,7 @@ struct AsmArgs {
     const int32_t* el_conn;    // [4][n_elem]
     const uint8_t* el_mat;     // [n_elem]
     const int32_t* cut_index;  // [n_elem], -1 if uncut
+    const uint8_t* image;      // [n_all] or null: 1 if the node's dofs belong to an earlier node (periodic image)
     const int64_t* inc_ptr;    // [n_all+1]
     const int32_t* inc_elem;
     const uint8_t* inc_slot;

@@ -42,10 +42,11 @@ def cohesive_problem(n, n_incl, seed, law=configs.BILINEAR_C2, r_lo=0.06, r_hi=0
 
 
 def run_steps(prob, incl, n_steps, frac_step, use_guess=0, net=None, record=False, lin_rtol=1e-10, dt=None,
-              precision=64, callback=None, ctx=None, adaptive_lin=False, fp32_maxit=0):
+              precision=64, callback=None, ctx=None, adaptive_lin=False, fp32_maxit=0, keep_solutions=None):
     """Solve load steps frac = k * frac_step, k = 1..n_steps from the unloaded
     state; returns per-step records, the damage flags after every step and the
-    context (pass it back in to reuse its device set-up)."""
+    context (pass it back in to reuse its device set-up).  keep_solutions, a
+    list, receives the nodal displacement after every step."""
     if ctx is None:
         ctx = DeviceContext(prob)
     else:
@@ -70,6 +71,8 @@ def run_steps(prob, incl, n_steps, frac_step, use_guess=0, net=None, record=Fals
                    guess=g, fp64_fallback=si.fp64_fallback)
         out.append(rec)
         flags.append(ctx.isv()[2].copy())
+        if keep_solutions is not None:
+            keep_solutions.append(ctx.solution())
         if callback:
             callback(rec)
     return out, flags, ctx

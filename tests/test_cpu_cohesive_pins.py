@@ -1,7 +1,7 @@
 """Pins of the oracle's cohesive element assembly and enriched gradients
 against what the mathematics fixes (no GPU).
 
-Cohesive assembly (src/Assembler.cpp:583-670, src/Utilities.cpp:465-531,
+Cohesive assembly (src/Assembler.cpp:436-524, src/Utilities.cpp:465-531,
 src/CohesiveElem.cpp:207-242):
 
 * uniform jump: every upper (mirror) node of the interface displaced by d,

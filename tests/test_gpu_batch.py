@@ -95,3 +95,105 @@ def test_effective_stiffness_closed_forms():
     print("C_eff\n%s\nVoigt\n%s\nReuss\n%s" % (Cs, voigt, reuss))
     assert np.min(np.linalg.eigvalsh(voigt - Cs)) > 0
     assert np.min(np.linalg.eigvalsh(Cs - reuss)) > 0
+
+
+def test_batch_solves_match_oracle_fp64():
+    """Five RVEs of 64^2 to 128^2: each RVE's three load-case solutions from the
+    segmented batched PCG against the oracle's FP64 solve of that RVE alone
+    (reference-structured assembly, PCG to 1e-14), to 1e-8 relative."""
+    from oracle import oracle as orc
+    sizes = [(64, 31), (80, 32), (96, 33), (112, 34), (128, 35)]
+    parts = [batch.rve_problem(N, seed)[0] for N, seed in sizes]
+    B = batch.BatchProblem(parts)
+    ctx = DeviceContext(B)
+    ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS)
+    UP = B.case_values()
+    X, its, rel, ms, tune = ctx.batch_pcg(B.sys_ptr, ctx.rhs_prescribed(UP), rtol=1e-12)
+    assert np.all(rel <= 1e-12)
+    for k, p in enumerate(parts):
+        dof, _ = p.dofs()
+        nf = p.info.n_free
+        Xr = X[:, B.sys_ptr[k]:B.sys_ptr[k + 1]]
+        coords = p.coords()
+        for c, E in enumerate(batch.CASES):
+            bcs = [(n, d, float(E[d] @ coords[:2, n])) for n, d, _ in p.bc_list()]
+            o = orc.Oracle(p, bcs)
+            it, info = o.newton_step(1.0, lin_rtol=1e-14)
+            assert it >= 1
+            Uo = o.solution()
+            uo = np.zeros(nf)
+            for d in range(2):
+                free = dof[d] < nf
+                uo[dof[d][free]] = Uo[d][free]
+            err = np.max(np.abs(Xr[c] - uo)) / np.max(np.abs(uo))
+            print("RVE %d (N=%d) case %d: %d batched iterations, rel err vs oracle %.3e" % (
+                k, sizes[k][0], c, its[k, c], err))
+            assert err <= 1e-8
+
+
+def test_periodic_rve_matches_oracle():
+    """One periodic RVE (the reference's linear subscale model): pattern and K
+    against the oracle, the macro-strain body load (device: minus the internal
+    force of the affine field; oracle: BF_Kernel per point) to 1e-12, the
+    Newton solve to 1e-8 and the volume-averaged stress to 1e-10."""
+    from oracle import oracle as orc
+    from parity_bars import k_entries_ok
+    p, _ = batch.periodic_rve_problem(48, 7)
+    o = orc.Oracle(p, p.bc_list())
+    ctx = DeviceContext(p)
+    rp, rs, ci = ctx.pattern()
+    orp, oci = o.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS)
+    assert k_entries_ok(ctx.matrix(), o, rp, "periodic RVE")
+    nf = p.info.n_free
+    F = batch.macro_strain_rhs(ctx, p)
+    for k, e in enumerate(batch.VOIGT_CASES):
+        fo = o.macro_strain_load(e)[:nf]
+        assert np.max(np.abs(F[k] - fo)) <= 1e-12 * np.max(np.abs(fo))
+        o.set_fext(fo)
+        o.set_solution(np.zeros((2, p.info.n_all_nodes)))
+        o.newton_step(1.0, lin_rtol=1e-14)
+        sig_o, eps_o, vol_o = o.homogenize(e)
+        ctx.set_solution(np.zeros((2, p.info.n_all_nodes)))
+        ctx.set_load(fext=F[k])
+        si = ctx.newton_step(1.0, rel_tol=1e-8, lin_rtol=1e-12, lin_maxit=100000)
+        assert si.converged
+        Ug, Uo = ctx.solution(), o.solution()
+        err = np.max(np.abs(Ug - Uo)) / np.max(np.abs(Uo))
+        sig, eps, vol = ctx.volume_average(e)
+        serr = np.max(np.abs(sig[0] - sig_o)) / np.max(np.abs(sig_o))
+        print("periodic case %d: newton %d pcg %d, U rel err %.3e, sigma_bar rel err %.3e, volume %.15f" % (
+            k, si.newton_iterations, si.pcg_iterations, err, serr, vol[0]))
+        assert err <= 1e-8
+        # the volume (sum of J*W) differs from 1 by the pentagon fan rule's error and
+        # between host and oracle tables by their rounding (independent inverse maps)
+        assert serr <= 1e-10 and abs(vol[0] - vol_o) <= 1e-12
+        assert np.max(np.abs(eps[0] - eps_o)) <= 1e-9 * np.max(np.abs(e))
+
+
+def test_periodic_batch_effective_stiffness_matches_oracle():
+    """Four periodic RVEs in one batch: the device macro-strain right-hand
+    sides, one segmented batched PCG for the three load cases and the
+    per-RVE volume averages give each RVE's C_eff to 1e-8 of the oracle's."""
+    from oracle import oracle as orc
+    sizes = [(40, 41), (56, 42), (64, 43), (72, 44)]
+    parts = [batch.periodic_rve_problem(N, seed)[0] for N, seed in sizes]
+    B = batch.BatchProblem(parts)
+    ctx = DeviceContext(B)
+    ctx.assemble(L.PGF_ASM_JAC)
+    RHS = batch.macro_strain_rhs(ctx, B)
+    X, its, rel, ms, tune = ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-12)
+    assert np.all(rel <= 1e-12)
+    Ce = batch.homogenized_stiffness(ctx, B, X, B.elem_ptr)
+    for k, p in enumerate(parts):
+        Co = np.zeros((3, 3))
+        for c, e in enumerate(batch.VOIGT_CASES):
+            o = orc.Oracle(p, p.bc_list())
+            f = o.macro_strain_load(e)
+            o.set_fext(f[: o.n_free])
+            o.newton_step(1.0, lin_rtol=1e-14)
+            Co[:, c] = o.homogenize(e)[0]
+        err = np.max(np.abs(Ce[k] - Co)) / np.max(np.abs(Co))
+        print("RVE %d (N=%d): batched iterations %s, C_eff rel err vs oracle %.3e" % (k, sizes[k][0], its[k], err))
+        assert err <= 1e-8

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import cnn_oracle as N
-from pigfem_assignment_amd.api import GuessNet
+from pigfem_assignment_amd.api import DeviceContext, GuessNet
 
 pytestmark = pytest.mark.gpu
 
@@ -115,3 +115,41 @@ def test_learned_guess_reduces_pcg_work():
     tot = {m: sum(r["pcg"] for r in res[m][0]) for m in res}
     print("total PCG iterations: plain %d, previous increment %d, learned %d" % (tot[0], tot[3], tot[2]))
     assert tot[2] < tot[0]
+
+
+@pytest.mark.parametrize("use_guess", [2, 3])
+def test_learned_guess_steps_match_oracle(use_guess):
+    """Load steps started from the learned guess (network + previous increment,
+    use_guess=2) or from the previous increment alone (3) converge to the
+    oracle's FP64 solution: U within 1e-8, damage flags bit-exact, on a small
+    3D cohesive problem with damage growth."""
+    from oracle import oracle as orc
+    from pigfem_assignment_amd import configs, learn
+
+    prob = configs.c2_problem(n=12, n_incl=2, r_lo=0.09, r_hi=0.11, strain=0.2)
+    incl = configs.c2_inclusions(2, 12, configs.SEED, 0.09, 0.11)
+    net = GuessNet(seed=7)
+    learn.train_net(net, [(prob, incl)], 3, 0.05, epochs=5, lr=1e-3)
+    o = orc.Oracle(prob, prob.bc_list())
+    recs, flags, ctx = learn.run_steps(prob, incl, 4, 0.05, use_guess=use_guess, net=net, lin_rtol=1e-12)
+    # the trajectory of device solutions: replay against the oracle step by step
+    ctx2 = DeviceContext(prob)
+    ctx2.learn_attach(net, indicator=learn.indicator(incl))
+    grew = False
+    for k in range(1, 5):
+        si = ctx2.newton_step(0.05 * k, dt=0.05, use_guess=use_guess, lin_rtol=1e-12)
+        assert si.converged and si.res <= 1e-8 * si.res0
+        ctx2.isv_commit()
+        ctx2.learn_commit()
+        it, _ = o.newton_step(0.05 * k, dt=0.05, lin_rtol=1e-14)
+        assert it >= 0
+        o.isv_commit()
+        err = np.max(np.abs(ctx2.solution() - o.solution())) / np.max(np.abs(o.solution()))
+        fl_g, fl_o = ctx2.isv()[2].astype(np.int32), o.isv()[2]
+        grew = grew or bool(np.any(fl_o & 8))
+        print("use_guess %d step %d: newton %d pcg %d, U rel err vs oracle %.3e" % (
+            use_guess, k, si.newton_iterations, si.pcg_iterations, err))
+        assert err <= 1e-8
+        assert np.array_equal(fl_g, fl_o)
+        assert np.array_equal(fl_g.astype(np.uint8), flags[k - 1])
+    assert grew  # the steps include softening (delta_max > delta_c somewhere)

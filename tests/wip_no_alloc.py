@@ -1,0 +1,46 @@
+"""The allocation-free entry point pgf_volume_average_noalloc (device scratch
+in a static array per GPU) against pgf_volume_average: bit-identical results
+on two calls in a row, the second with fewer segments, and a clear error
+beyond the maximum segment count."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from pigfem_assignment_amd import batch, _lib as L
+from pigfem_assignment_amd.api import DeviceContext
+
+pytestmark = pytest.mark.gpu
+
+
+def _noalloc(ctx, eps, seg):
+    lib = L.lib()
+    lib.pgf_volume_average_noalloc.argtypes = [C.c_void_p, L.dp, C.c_int32, L.i64p, L.dp]
+    e = np.ascontiguousarray(eps, np.float64)
+    s = np.ascontiguousarray(seg, np.int64)
+    out = np.full((len(s) - 1, 2 * len(e) + 1), np.nan)
+    rc = lib.pgf_volume_average_noalloc(ctx.h, L.ptr(e, C.c_double), len(s) - 1,
+                                        s.ctypes.data_as(C.POINTER(C.c_int64)), L.ptr(out, C.c_double))
+    return rc, out
+
+
+def test_noalloc_volume_average_matches_allocating_entry_point():
+    parts = [batch.periodic_rve_problem(N, seed)[0] for N, seed in ((24, 61), (32, 62), (40, 63))]
+    B = batch.BatchProblem(parts)
+    ctx = DeviceContext(B)
+    ctx.assemble(L.PGF_ASM_JAC)
+    X, _, rel, _, _ = ctx.batch_pcg(B.sys_ptr, batch.macro_strain_rhs(ctx, B), rtol=1e-12)
+    ctx.set_solution_dofs(X[0])
+    e = batch.VOIGT_CASES[0]
+    for seg in (B.elem_ptr, B.elem_ptr[:2]):
+        rc, got = _noalloc(ctx, e, seg)
+        assert rc == 0
+        sig, eps, vol = ctx.volume_average(e, seg)
+        ref = np.concatenate([sig, eps, vol[:, None]], axis=1)
+        assert np.array_equal(got, ref)
+        print("segments %d: noalloc == allocating, volumes %s" % (len(seg) - 1, vol))
+    # beyond the maximum: a clear error, nothing written
+    big = np.linspace(0, B.n_elem, 1026).astype(np.int64)
+    rc, out = _noalloc(ctx, e, big)
+    assert rc == L.PGF_ELIMIT if hasattr(L, "PGF_ELIMIT") else rc == -7
+    assert "at most 1024" in ctx.lib.pgf_last_error(ctx.h).decode()
