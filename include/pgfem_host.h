@@ -106,6 +106,36 @@ PGF_API int pgfh_get_cut(pgfh_problem* p, int32_t* rec);
 PGF_API int pgfh_shape_sensitivity(pgfh_problem* p, int32_t as_reference, double* dgrad, double* divv, int32_t* param,
                                    double* vel);
 
+/* legacy ASCII VTK file of the mesh (cut elements as their integration
+ * sub-elements), the displacement at every node (the physical value at
+ * enrichment nodes, plus eps_bar x when eps_bar [3 or 6] is given) and the
+ * material of every cell (Writer_ASCII_VTK / Writer_VTK,
+ * src/Writer_ASCII_VTK.cpp:43-282, src/Writer_VTK.cpp:221-335, 639-730).
+ * u_nodal [dim][n_all_nodes] may be NULL (mesh only). */
+PGF_API int pgfh_write_vtk(pgfh_problem* p, const char* path, const double* u_nodal, const double* eps_bar, double t);
+
+/* ---- batches of RVEs ----
+ * Independent problems stacked into one block-diagonal problem for one
+ * pgf_create and pgf_batch_pcg.  The reference has no batch: it solves each
+ * RVE as its own Problem (SubscaleModel::assemble / solve,
+ * src/SubscaleModel.cpp:42-196, once per macro point).  Regular nodes of all
+ * parts come first (part order), then all enrichment nodes; part k's free
+ * dofs are [free_ptr[k], free_ptr[k+1]) (its segment in pgf_batch_pcg), its
+ * prescribed dofs n_free + [pres_ptr[k], pres_ptr[k+1]).  Parts must share
+ * dimension, element kind and materials; cohesive parts are refused
+ * (PGF_EUNSUPPORTED).  The parts may be destroyed after pgfh_batch_create. */
+typedef struct pgfh_batch pgfh_batch;
+PGF_API int pgfh_batch_create(pgfh_problem* const* parts, int32_t n_parts, pgfh_batch** out);
+PGF_API void pgfh_batch_destroy(pgfh_batch* b);
+/* descriptor pointing into the batch's storage (valid while b lives) */
+PGF_API int pgfh_batch_desc(pgfh_batch* b, pgf_mesh_desc* d);
+/* [n_parts+1] offsets each (any may be NULL): free dofs, prescribed dofs,
+ * elements, regular nodes, enrichment nodes */
+PGF_API int pgfh_batch_offsets(pgfh_batch* b, int64_t* free_ptr, int64_t* pres_ptr, int64_t* elem_ptr,
+                               int64_t* reg_ptr, int64_t* enr_ptr);
+/* [dim][n_all_nodes] batch dof ids, [n_pres] prescribed values */
+PGF_API int pgfh_batch_get_dofs(pgfh_batch* b, int32_t* dof, double* pres);
+
 /* CSR pattern and element colouring exactly as pgf_create builds them (host
  * integer work, no device needed): sizes first, then the arrays */
 PGF_API int pgfh_pattern_info(pgfh_problem* p, int64_t* n_rows, int64_t* nnz, int32_t* n_colors);

@@ -108,7 +108,8 @@ EXPORTS = [
     "pgf_net_train", "pgf_learn_attach", "pgf_learn_detach", "pgf_learn_commit", "pgf_learn_record", "pgf_learn_last",
     "pgf_learn_features", "pgf_batch_pcg", "pgf_rhs_prescribed", "pgf_reactions", "pgf_set_solution_dofs",
     "pgf_volume_average", "pgf_volume_average_noalloc", "pgfh_shape_sensitivity", "pgf_set_shape_sensitivity",
-    "pgf_shape_pseudo_load",
+    "pgf_shape_pseudo_load", "pgfh_write_vtk", "pgfh_batch_create", "pgfh_batch_destroy", "pgfh_batch_desc",
+    "pgfh_batch_offsets", "pgfh_batch_get_dofs",
 ]
 PGF_ELIMIT = -7
 

@@ -153,6 +153,17 @@ class HostProblem:
         L.check(L.lib().pgfh_mesh_desc(self.h, C.byref(d)), host=True)
         return d
 
+    def write_vtk(self, path, U=None, eps_bar=None, t=0.0):
+        """legacy ASCII VTK file: mesh (cut elements as their sub-elements),
+        displacement (physical at enrichment nodes; plus eps_bar x if given),
+        material per cell (pgfh_write_vtk)"""
+        lib = L.lib()
+        lib.pgfh_write_vtk.argtypes = [C.c_void_p, C.c_char_p, L.dp, L.dp, C.c_double]
+        u = None if U is None else np.ascontiguousarray(U, np.float64)
+        e = None if eps_bar is None else np.ascontiguousarray(eps_bar, np.float64)
+        L.check(lib.pgfh_write_vtk(self.h, str(path).encode(), L.ptr(u, C.c_double), L.ptr(e, C.c_double), float(t)),
+                host=True)
+
 
 class DeviceContext:
     """One mesh resident on the GPU (pgf_create)."""

@@ -78,128 +78,52 @@ def periodic_rve_problem(N, seed, materials=(configs.MATRIX, configs.INCLUSION),
                        periodic=True), incl
 
 
-def _arr(ptr, n, dtype):
-    if n == 0:
-        return np.zeros(0, dtype)
-    return np.ctypeslib.as_array(ptr, shape=(int(n),)).astype(dtype, copy=True)
-
-
 class BatchProblem:
     """Concatenation of independent HostProblems (same dimension, materials,
-    element kind), exposing what DeviceContext needs: desc() and dofs()."""
+    element kind) built by the library (pgfh_batch_create), exposing what
+    DeviceContext needs: desc() and dofs()."""
 
     def __init__(self, problems):
         self.parts = problems
-        descs = [p.desc() for p in problems]
-        d0 = descs[0]
-        dim = d0.dim
-        self.dim = dim
-        n_reg = [d.n_nodes for d in descs]
-        n_enr = [d.n_all_nodes - d.n_nodes for d in descs]
-        NR, NE = sum(n_reg), sum(n_enr)
-        n_all = NR + NE
-        R_off = np.concatenate([[0], np.cumsum(n_reg)])
-        E_off = np.concatenate([[0], np.cumsum(n_enr)])
-        nf = [d.n_free for d in descs]
-        npr = [d.n_pres for d in descs]
-        F_off = np.concatenate([[0], np.cumsum(nf)])
-        P_off = np.concatenate([[0], np.cumsum(npr)])
-        NF, NP = int(F_off[-1]), int(P_off[-1])
-        self.sys_ptr = F_off.astype(np.int64)
-        self.pres_ptr = P_off.astype(np.int64)
-        coords = np.zeros((3, n_all))
-        dof = np.zeros((dim, n_all), np.int32)
-        conn, emat, cut_elem, cut_ne, cut_en = [], [], [], [], []
-        vpt_ptr, vpt_w, vpt_mat, vpt_goff, vpt_grad = [np.zeros(1, np.int64)], [], [], [], []
-        e_off = v_off = g_off = 0
-        pres = []
-        for k, (p, d) in enumerate(zip(problems, descs)):
-            na = d.n_all_nodes
-            X = _arr(d.coords, 3 * na, np.float64).reshape(3, na)
-            reg = slice(R_off[k], R_off[k + 1])
-            enr = slice(NR + E_off[k], NR + E_off[k + 1])
-            coords[:, reg] = X[:, : d.n_nodes]
-            coords[:, enr] = X[:, d.n_nodes:]
-            dk = _arr(d.dof, dim * na, np.int32).reshape(dim, na)
-            newd = np.where(dk < d.n_free, dk + F_off[k], NF + P_off[k] + (dk - d.n_free)).astype(np.int32)
-            dof[:, reg] = newd[:, : d.n_nodes]
-            dof[:, enr] = newd[:, d.n_nodes:]
-            conn.append((_arr(d.conn, 4 * d.n_elem, np.int32) + R_off[k]).astype(np.int32))
-            emat.append(_arr(d.elem_mat, d.n_elem, np.int32))
-            nc = d.n_cut
-            cut_elem.append((_arr(d.cut_elem, nc, np.int32) + e_off).astype(np.int32))
-            cut_ne.append(_arr(d.cut_n_enrich, nc, np.int32))
-            ce = _arr(d.cut_enrich, 16 * nc, np.int32).reshape(max(nc, 0), 16)
-            ne = cut_ne[-1]
-            mask = np.arange(16)[None, :] < ne[:, None]
-            ce = np.where(mask, ce + E_off[k], ce).astype(np.int32)
-            cut_en.append(ce.ravel())
-            vp = _arr(d.vpt_ptr, nc + 1, np.int64) if nc else np.zeros(1, np.int64)
-            nv = int(vp[-1])
-            vpt_ptr.append((vp[1:] + v_off).astype(np.int64))
-            vpt_w.append(_arr(d.vpt_w, nv, np.float64))
-            vpt_mat.append(_arr(d.vpt_mat, nv, np.int32))
-            go = _arr(d.vpt_grad_off, nv, np.int64)
-            ng = int(p.info.n_vgrad)
-            vpt_goff.append((go + g_off).astype(np.int64))
-            vpt_grad.append(_arr(d.vpt_grad, ng, np.float64))
-            if p.info.n_cpt > 0:
-                raise ValueError("cohesive RVEs are not batched")
-            e_off += d.n_elem
-            v_off += nv
-            g_off += ng
-            pres.append(p.dofs()[1])
-        self.n_nodes, self.n_all_nodes, self.n_elem = NR, n_all, e_off
-        self.elem_ptr = np.concatenate([[0], np.cumsum([d.n_elem for d in descs])]).astype(np.int64)
-        self.reg_ptr = R_off.astype(np.int64)
-        self.n_free, self.n_pres = NF, NP
-        keep = dict(
-            coords=np.ascontiguousarray(coords.ravel()), dof=np.ascontiguousarray(dof.ravel()),
-            conn=np.ascontiguousarray(np.concatenate(conn)), emat=np.ascontiguousarray(np.concatenate(emat)),
-            cut_elem=np.ascontiguousarray(np.concatenate(cut_elem)), cut_ne=np.ascontiguousarray(np.concatenate(cut_ne)),
-            cut_en=np.ascontiguousarray(np.concatenate(cut_en)), vpt_ptr=np.ascontiguousarray(np.concatenate(vpt_ptr)),
-            vpt_w=np.ascontiguousarray(np.concatenate(vpt_w)), vpt_mat=np.ascontiguousarray(np.concatenate(vpt_mat)),
-            vpt_goff=np.ascontiguousarray(np.concatenate(vpt_goff)),
-            vpt_grad=np.ascontiguousarray(np.concatenate(vpt_grad)),
-            cpt_ptr=np.zeros(len(np.concatenate(cut_elem)) + 1, np.int64),
-            mat_E=np.ascontiguousarray([m[0] for m in problems[0].materials], np.float64),
-            mat_nu=np.ascontiguousarray([m[1] for m in problems[0].materials], np.float64),
-            pres=np.ascontiguousarray(np.concatenate(pres)))
-        self._keep = keep
-        n_cut = len(keep["cut_elem"])
-        self.n_cut = n_cut
+        lib = L.lib()
+        lib.pgfh_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
+        lib.pgfh_batch_destroy.argtypes = [C.c_void_p]
+        lib.pgfh_batch_destroy.restype = None
+        lib.pgfh_batch_desc.argtypes = [C.c_void_p, C.POINTER(L.MeshDesc)]
+        lib.pgfh_batch_offsets.argtypes = [C.c_void_p] + [L.i64p] * 5
+        lib.pgfh_batch_get_dofs.argtypes = [C.c_void_p, L.i32p, L.dp]
+        arr = (C.c_void_p * len(problems))(*[p.h for p in problems])
+        h = C.c_void_p()
+        L.check(lib.pgfh_batch_create(arr, len(problems), C.byref(h)), host=True)
+        self.h = h
         D = L.MeshDesc()
-        D.dim, D.elem_kind, D.plane_strain = dim, d0.elem_kind, d0.plane_strain
-        D.n_nodes, D.n_all_nodes = NR, n_all
-        D.coords = L.ptr(keep["coords"], C.c_double)
-        D.n_elem = e_off
-        D.conn = L.ptr(keep["conn"], C.c_int32)
-        D.elem_mat = L.ptr(keep["emat"], C.c_int32)
-        D.n_mat = len(keep["mat_E"])
-        D.n_coh_law = 0
-        D.mat_E = L.ptr(keep["mat_E"], C.c_double)
-        D.mat_nu = L.ptr(keep["mat_nu"], C.c_double)
-        D.n_cut = n_cut
-        D.cut_elem = L.ptr(keep["cut_elem"], C.c_int32)
-        D.cut_n_enrich = L.ptr(keep["cut_ne"], C.c_int32)
-        D.cut_enrich = L.ptr(keep["cut_en"], C.c_int32)
-        D.vpt_ptr = L.ptr(keep["vpt_ptr"], C.c_int64)
-        D.vpt_w = L.ptr(keep["vpt_w"], C.c_double)
-        D.vpt_mat = L.ptr(keep["vpt_mat"], C.c_int32)
-        D.vpt_grad_off = L.ptr(keep["vpt_goff"], C.c_int64)
-        D.vpt_grad = L.ptr(keep["vpt_grad"], C.c_double)
-        D.cpt_ptr = L.ptr(keep["cpt_ptr"], C.c_int64)
-        D.n_free, D.n_pres = NF, NP
-        D.dof = L.ptr(keep["dof"], C.c_int32)
+        L.check(lib.pgfh_batch_desc(h, C.byref(D)), host=True)
         self._desc = D
-        self.coords_full = coords
-        self.dof_full = dof
+        self.dim = D.dim
+        self.n_nodes, self.n_all_nodes, self.n_elem = D.n_nodes, D.n_all_nodes, D.n_elem
+        self.n_free, self.n_pres, self.n_cut = D.n_free, D.n_pres, D.n_cut
+        offs = [np.zeros(len(problems) + 1, np.int64) for _ in range(5)]
+        L.check(lib.pgfh_batch_offsets(h, *[L.ptr(o, C.c_int64) for o in offs]), host=True)
+        self.sys_ptr, self.pres_ptr, self.elem_ptr, self.reg_ptr, self.enr_ptr = offs
+        self.dof_full = np.zeros((self.dim, self.n_all_nodes), np.int32)
+        self._pres = np.zeros(max(self.n_pres, 1))
+        L.check(lib.pgfh_batch_get_dofs(h, L.ptr(self.dof_full, C.c_int32), L.ptr(self._pres, C.c_double)),
+                host=True)
+        self._pres = self._pres[: self.n_pres]
+        self.coords_full = np.ctypeslib.as_array(D.coords, shape=(3 * self.n_all_nodes,)).reshape(
+            3, self.n_all_nodes).copy()
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            L.lib().pgfh_batch_destroy(h)
+            self.h = None
 
     def desc(self):
         return self._desc
 
     def dofs(self):
-        return self.dof_full, self._keep["pres"]
+        return self.dof_full, self._pres
 
     def case_values(self):
         """Prescribed values of the three load cases, [3][n_pres], from the

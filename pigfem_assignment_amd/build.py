@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpgfem.so")
 
-HOST_SRC = ["host/pgf_mesh.cpp", "host/pgf_dofs.cpp", "host/pgf_problem.cpp", "host/pgf_sens.cpp"]
+HOST_SRC = ["host/pgf_mesh.cpp", "host/pgf_dofs.cpp", "host/pgf_problem.cpp", "host/pgf_sens.cpp",
+            "host/pgf_writer.cpp", "host/pgf_batch_host.cpp"]
 HIP_SRC = ["kernels/pgf_assemble.hip", "kernels/pgf_device.hip", "kernels/pgf_cnn.hip", "kernels/pgf_batch.hip",
            "kernels/pgf_sens.hip"]
 

@@ -24,6 +24,11 @@ namespace {
 std::string g_host_error;
 }
 
+namespace pgf {
+void set_host_error(const std::string& msg) { g_host_error = msg; }
+void write_vtk(const Mesh& m, const std::string& path, const double* u, const double* eps_bar, double t);
+}
+
 struct pgfh_problem {
     pgf::Mesh mesh;
     pgf::DofMap dofs;
@@ -257,6 +262,17 @@ PGF_API int pgfh_shape_sensitivity(pgfh_problem* p, int32_t as_reference, double
         std::memcpy(divv, t.divv.data(), sizeof(double) * t.divv.size());
         std::memcpy(param, t.param.data(), sizeof(int32_t) * t.param.size());
         if (vel) std::memcpy(vel, t.vel.data(), sizeof(double) * t.vel.size());
+        return 0;
+    } catch (const pgf::Error& e) {
+        g_host_error = e.what();
+        return e.code;
+    }
+}
+
+PGF_API int pgfh_write_vtk(pgfh_problem* p, const char* path, const double* u_nodal, const double* eps_bar, double t) {
+    if (!p || !path) return PGF_EARG;
+    try {
+        pgf::write_vtk(p->mesh, path, u_nodal, eps_bar, t);
         return 0;
     } catch (const pgf::Error& e) {
         g_host_error = e.what();

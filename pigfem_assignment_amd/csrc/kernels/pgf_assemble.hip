@@ -528,6 +528,228 @@ __global__ __launch_bounds__(cut_threads<DIM>()) void k_cut_elem(AsmArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// 3D cut-element matrices on the FP64 matrix cores.
+//
+// The element matrix of a cut tet (src/Assembler.cpp:392-524: volumetric
+// sub-element points, then cohesive points) is one small dense product
+//     K_e = A^T C,
+// with the rows of A and C stacked over the element's integration points:
+//   * volumetric point q (6 rows, Voigt s):  A = B_q,   C = w_q D_q B_q
+//     (B^T D B of SmallStrainKFast, src/Utilities.cpp:196-296);
+//   * cohesive point p (3 rows, jump component d):  A = L_p,  C = w_p T_p L_p
+//     with L_p (u) = sum_n N_n (u_upper,n - u_lower,n) and T_p = R^T D_c R
+//     (KernelCohesive / cohesiveKFast, src/AssemblerSmallStrainNonlinear.cpp:73-91,
+//     src/Utilities.cpp:480-508: the +-N_i N_j R^T D R blocks).
+// One wavefront per cut element, four per block.  Each lane forms the A and C
+// operand values it feeds to v_mfma_f64_16x16x4f64 straight from the point data
+// staged in LDS (gradients, weights, isotropic D, R^T D_c R, signed surface
+// shape functions), so no operand matrix is materialised; K_e (up to 36 x 36,
+// 2 x 2 or 3 x 3 output tiles of 16 x 16) accumulates in registers and is
+// written once to the cut-element scratch the gather pass reads.  P_e and the
+// cohesive law (trial delta_max and damage flags) are as in cut_compute.
+//
+// v_mfma_f64_16x16x4f64 lane layout (wave64): A operand lane l holds
+// A(row l % 16, k l / 16); B operand lane l holds B(k l / 16, col l % 16); the
+// four results of lane l are D(4 r + l / 16, l % 16), r = 0..3 (rows
+// interleaved across the four lane groups, unlike the fp32 16x16x4 form).
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int MF_WAVES = 4;                    // cut elements per block
+constexpr int MF_NV = 16;                      // volumetric points staged (max_vpt(3))
+constexpr int MF_MM = MAX_NT * 3;              // 36
+
+struct MfWave {
+    double g[MF_NV * MF_MM];      // slot gradients per point
+    double w[MF_NV];
+    double Dabc[MF_NV * 3];       // isotropic D: a (normal), b (off-diagonal), c (shear)
+    double sig[MF_NV * 6];
+    double u[MF_MM];
+    double T[MAX_CPT * 9];        // w_p R^T D_c R
+    double cs[MAX_CPT * MAX_NT];  // signed surface shape function of every slot (upper +N, lower -N)
+    double ctR[MAX_CPT * 3];      // w_p R^T t
+    int node[MAX_NT];
+    int nv, nc, nt;
+};
+
+// B_q(s, (a, da)) of a volumetric point from its slot gradient ga
+__device__ inline double b_entry(const double* ga, int s, int da) {
+    if (s < 3) return s == da ? ga[da] : 0.0;
+    if (s == 3) return da == 1 ? ga[2] : (da == 2 ? ga[1] : 0.0);
+    if (s == 4) return da == 0 ? ga[2] : (da == 2 ? ga[0] : 0.0);
+    return da == 0 ? ga[1] : (da == 1 ? ga[0] : 0.0);
+}
+// (D B_q)(s, (b, db)) for the isotropic D (a, b, c)
+__device__ inline double db_entry(const double* gb, int s, int db, double a, double b, double c) {
+    if (s < 3) return (s == db ? a : b) * gb[db];
+    return c * b_entry(gb, s, db);
+}
+
+// A(kk, i) and C(kk, j) of the stacked point rows (zero beyond the element's rows)
+__device__ inline double a_val(const MfWave& W, int kk, int i, int M) {
+    if (i >= M) return 0.0;
+    const int a = i / 3, da = i - 3 * a;
+    const int kv = 6 * W.nv;
+    if (kk < kv) {
+        const int q = kk / 6, s = kk - 6 * q;
+        return b_entry(&W.g[q * MF_MM + a * 3], s, da);
+    }
+    const int kc = kk - kv;
+    if (kc >= 3 * W.nc) return 0.0;
+    const int p = kc / 3, d = kc - 3 * p;
+    return d == da ? W.cs[p * MAX_NT + a] : 0.0;
+}
+__device__ inline double c_val(const MfWave& W, int kk, int j, int M) {
+    if (j >= M) return 0.0;
+    const int b = j / 3, db = j - 3 * b;
+    const int kv = 6 * W.nv;
+    if (kk < kv) {
+        const int q = kk / 6, s = kk - 6 * q;
+        return W.w[q] * db_entry(&W.g[q * MF_MM + b * 3], s, db, W.Dabc[3 * q], W.Dabc[3 * q + 1], W.Dabc[3 * q + 2]);
+    }
+    const int kc = kk - kv;
+    if (kc >= 3 * W.nc) return 0.0;
+    const int p = kc / 3, d = kc - 3 * p;
+    return W.T[p * 9 + d * 3 + db] * W.cs[p * MAX_NT + b];
+}
+
+template <int NT>
+__device__ inline void mfma_ke(const MfWave& W, int M, int lane, double* ke) {
+    v4d acc[NT][NT];
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int KD = 6 * W.nv + 3 * W.nc;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    for (int k0 = 0; k0 < KD; k0 += 4) {
+        const int kk = k0 + k4;
+        double av[NT], cv[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            av[t] = kk < KD ? a_val(W, kk, 16 * t + r16, M) : 0.0;
+            cv[t] = kk < KD ? c_val(W, kk, 16 * t + r16, M) : 0.0;
+        }
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+                acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], cv[tj], acc[ti][tj], 0, 0, 0);
+    }
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti + 4 * r + k4, col = 16 * tj + r16;
+                if (row < M && col < M) ke[row * M + col] = acc[ti][tj][r];
+            }
+}
+
+template <bool DO_K, bool DO_F>
+__global__ __launch_bounds__(MF_WAVES * WAVE) void k_cut_elem_mfma(AsmArgs a) {
+    __shared__ MfWave s_w[MF_WAVES];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
+    const int64_t cl = blk * MF_WAVES + wave;
+    const bool live = cl < a.n_cut;  // padding waves keep to the barriers
+    MfWave& W = s_w[wave];
+    const int c = live ? (int)cl : 0;
+    const int nt = live ? a.ce_nt[c] : 0, M = 3 * nt;
+    const int64_t v0 = live ? a.vpt_ptr[c] : 0, c0 = live ? a.cpt_ptr[c] : 0;
+    const int nv = live ? (int)(a.vpt_ptr[c + 1] - v0) : 0;
+    const int nc = live ? (int)(a.cpt_ptr[c + 1] - c0) : 0;
+    if (lane == 0) { W.nv = nv; W.nc = nc; W.nt = nt; }
+    if (lane < nt) W.node[lane] = a.ce_nodes[c * 16 + lane];
+    __syncthreads();
+    for (int i = lane; i < M; i += WAVE) W.u[i] = a.U[(i % 3) * a.n_all + W.node[i / 3]];
+    for (int k = lane; k < nv * M; k += WAVE) {
+        const int q = k / M;
+        W.g[q * MF_MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
+    }
+    if (lane < nv) {
+        W.w[lane] = a.vpt_w[v0 + lane];
+        const double* D = a.Dmat + a.vpt_mat[v0 + lane] * 36;
+        W.Dabc[3 * lane] = D[0];
+        W.Dabc[3 * lane + 1] = D[1];
+        W.Dabc[3 * lane + 2] = D[35];
+    }
+    for (int k = lane; k < nc * MAX_NT; k += WAVE) W.cs[k] = 0.0;
+    __syncthreads();
+    if (DO_F && lane < nv) {
+        double eps[6];
+        strain_of<3>(nt, &W.g[lane * MF_MM], W.u, eps);
+        const double aa = W.Dabc[3 * lane], bb = W.Dabc[3 * lane + 1], cc = W.Dabc[3 * lane + 2];
+        double* s = &W.sig[lane * 6];
+        s[0] = aa * eps[0] + bb * eps[1] + bb * eps[2];
+        s[1] = bb * eps[0] + aa * eps[1] + bb * eps[2];
+        s[2] = bb * eps[0] + bb * eps[1] + aa * eps[2];
+        s[3] = cc * eps[3];
+        s[4] = cc * eps[4];
+        s[5] = cc * eps[5];
+    }
+    if (lane < nc) {
+        const int64_t q = c0 + lane;
+        const int nh = a.cpt_nh[q];
+        double R[9], dl[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
+        for (int k = 0; k < 9; ++k) R[k] = a.cpt_R[q * 9 + k];
+        for (int n = 0; n < 4; ++n) {
+            if (n >= nh) break;
+            const double Nn = a.cpt_N[q * 4 + n];
+            const int lo = a.cpt_nodes[q * 8 + n], hi = a.cpt_nodes[q * 8 + nh + n];
+            for (int d = 0; d < 3; ++d) dl[d] += Nn * (W.u[hi * 3 + d] - W.u[lo * 3 + d]);
+            W.cs[lane * MAX_NT + lo] = -Nn;
+            W.cs[lane * MAX_NT + hi] = Nn;
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) delta[i] += R[i * 3 + j] * dl[j];
+        double trac[3], Dc[9], dnew;
+        uint8_t fl;
+        coh_law<3>(a.laws[a.cpt_law[q]], delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
+        a.dmax_trial[q] = dnew;
+        a.flags[q] = fl;
+        const double wq = a.cpt_w[q];
+        double RtD[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += R[k * 3 + i] * Dc[k * 3 + j];
+                RtD[i * 3 + j] = s;
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += RtD[i * 3 + k] * R[k * 3 + j];
+                W.T[lane * 9 + i * 3 + j] = wq * s;
+            }
+        for (int i = 0; i < 3; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += R[k * 3 + i] * trac[k];
+            W.ctR[lane * 3 + i] = wq * s;
+        }
+    }
+    __syncthreads();
+    if (live && DO_K) {
+        double* ke = a.ke + a.ke_ptr[c];
+        if (M <= 32) mfma_ke<2>(W, M, lane, ke);
+        else mfma_ke<3>(W, M, lane, ke);
+    }
+    if (live && DO_F) {
+        double* pe = a.pe + a.pe_ptr[c];
+        for (int i = lane; i < M; i += WAVE) {
+            const int an = i / 3, d = i - 3 * an;
+            double acc = 0.0;
+#pragma unroll 1
+            for (int q = 0; q < nv; ++q) acc += W.w[q] * force_entry<3>(&W.g[q * MF_MM + an * 3], d, &W.sig[q * 6]);
+#pragma unroll 1
+            for (int p = 0; p < nc; ++p) acc += W.cs[p * MAX_NT + an] * W.ctR[p * 3 + d];
+            pe[i] = acc;
+        }
+    }
+}
+
+
 // Gather scheme, pass 2.  A block owns a run of consecutive nodes.  Its
 // threads are (unit, slot) pairs: a unit is one matrix row (NR = 1) or all DIM
 // rows of one node (NR = DIM, the element geometry and the nodal data then
@@ -815,9 +1037,17 @@ static void launch_colors(const Ctx& c, const pgfk::AsmArgs& a) {
 template <int DIM, bool DO_K, bool DO_F, typename KT>
 static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     if (c.timing_events) (void)hipEventRecord(c.timing_events[0], c.stream);
-    if (c.n_cut > 0)
-        hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(c.n_cut)),
-                           dim3(pgfk::cut_threads<DIM>()), 0, c.stream, a);
+    if (c.n_cut > 0) {
+        if (DIM == 3) {
+            // 3D: four cut elements per block, K_e on the FP64 matrix cores
+            const int64_t nb = (c.n_cut + pgfk::MF_WAVES - 1) / pgfk::MF_WAVES;
+            hipLaunchKernelGGL((pgfk::k_cut_elem_mfma<DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(nb)),
+                               dim3(pgfk::MF_WAVES * pgfk::WAVE), 0, c.stream, a);
+        } else {
+            hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(c.n_cut)),
+                               dim3(pgfk::cut_threads<DIM>()), 0, c.stream, a);
+        }
+    }
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
     const dim3 grid((unsigned)pgfk::xcd_blocks(c.n_gblk)), blk(pgfk::GATHER_THREADS);

@@ -43,7 +43,7 @@ def _rve(r, shift=0.0, N=24, r1=0.09):
 
 @pytest.mark.parametrize("which", [0, 1])
 def test_tables_match_central_differences(which):
-    r, r1, h = 0.27, 0.09, 1e-6
+    r, r1, h = 0.265, 0.09, 1e-6
     p0 = _rve(r, r1=r1)
     pp = _rve(r + h, r1=r1) if which == 0 else _rve(r, r1=r1 + h)
     pm = _rve(r - h, r1=r1) if which == 0 else _rve(r, r1=r1 - h)
@@ -80,7 +80,7 @@ def test_tables_match_central_differences(which):
 
 
 def test_reference_parent_term_differs_and_is_not_translation_invariant():
-    r = 0.27
+    r = 0.265
     ex0 = sens.host_tables(_rve(r))[0]
     rf0 = sens.host_tables(_rve(r), as_reference=True)[0]
     ex1 = sens.host_tables(_rve(r, shift=1.0))[0]

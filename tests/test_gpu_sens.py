@@ -7,8 +7,9 @@ oracle.
   radius moved by +-h (same topology and dof numbering), to 1e-5 of its
   largest entry.
 * dC_eff/dr of every inclusion of a periodic RVE: the direct method on the
-  device against the central difference of the oracle's C_eff (three FP64
-  solves per side), to 1e-5; and it is symmetric (C_eff is symmetric for
+  device against the central difference (h = 1e-6, truncation error about
+  2e-7 of the largest entry) of the oracle's C_eff (three FP64 solves per
+  side), to 1e-5; and it is symmetric (C_eff is symmetric for
   every radius).
 * A batch of periodic RVEs with different inclusion counts gives each RVE the
   sensitivities its own context gives (parameter slot j = its j-th
@@ -30,7 +31,15 @@ def _rve(incl, N=32):
                        materials=(configs.MATRIX, configs.INCLUSION), bcs=bcs, periodic=True)
 
 
-INCL = np.array([[0, 0.47, 0.52, 0.0, 0.27], [0, 0.16, 0.2, 0.0, 0.09]])
+INCL = np.array([[0, 0.47, 0.52, 0.0, 0.265], [0, 0.16, 0.2, 0.0, 0.09]])
+
+
+def _same_topology(p, q):
+    """a central difference is only the derivative while the cut topology
+    (enrichment nodes, sub-elements, dof numbering) stays the same: the radii
+    here keep every grid node at least 5e-4 away from the interfaces"""
+    assert p.info.n_all_nodes == q.info.n_all_nodes and p.info.n_free == q.info.n_free
+    assert np.array_equal(p.cut(), q.cut())
 
 
 def _oracle_C(p):
@@ -69,6 +78,7 @@ def test_pseudo_load_matches_central_difference_of_the_residual():
             inc = INCL.copy()
             inc[j, 4] += sgn * h
             q = _rve(inc)
+            _same_topology(p, q)
             o = orc.Oracle(q, q.bc_list())
             o.set_solution(U)
             _, P, _ = o.assemble()
@@ -87,13 +97,15 @@ def test_effective_stiffness_sensitivity_matches_central_difference():
     sp = np.array([0, p.info.n_free], np.int64)
     seg = np.array([0, p.info.n_elem], np.int64)
     dC, _ = sens.stiffness_sensitivities(ctx, X, seg, sp, 2, batch.VOIGT_CASES)
-    h = 1e-5
+    h = 1e-6
     for j in range(2):
         Cs = []
         for sgn in (1, -1):
             inc = INCL.copy()
             inc[j, 4] += sgn * h
-            Cs.append(_oracle_C(_rve(inc)))
+            q = _rve(inc)
+            _same_topology(p, q)
+            Cs.append(_oracle_C(q))
         fd = (Cs[0] - Cs[1]) / (2 * h)
         d = dC[0, j]
         err = np.max(np.abs(d - fd)) / np.max(np.abs(fd))
