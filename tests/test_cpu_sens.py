@@ -1,10 +1,11 @@
 """Shape-sensitivity tables of the host (pgfh_shape_sensitivity) against
 central differences of the host's own integration tables (no GPU).
 
-With the inclusion radius perturbed by +-h the mesh topology is unchanged
+With the inclusion radius perturbed by +-h, +-2h the mesh topology is unchanged
 (same cut elements, sub-elements and points), so every volumetric point of a
 cut element moves with its sub-element and the central difference of its
-tables is the material derivative the sensitivity needs:
+tables (Richardson-extrapolated, so the h^2 term cancels) is the material
+derivative the sensitivity needs, to 1e-8 of the largest entry:
   * enrichment-node positions  -> the intersection velocity v;
   * J*W of each point           -> J*W * div v;
   * gradients of every slot     -> the exact derivative dgrad (both the
@@ -32,7 +33,13 @@ def _tables(p):
     return vp, w, go, g
 
 
-def _rve(r, shift=0.0, N=24, r1=0.09):
+# radii whose interfaces keep clear of the nodes of the 24 x 24 grid (2.9e-4
+# or more; a node close to an interface cuts off a tiny sub-element, whose
+# tables vary fast with r and spoil finite differences)
+R0, R1 = 0.2664, 0.0936
+
+
+def _rve(r, shift=0.0, N=24, r1=R1):
     incl = np.array([[0, 0.47 + shift, 0.52 + shift, 0.0, r], [0, 0.16 + shift, 0.2 + shift, 0.0, r1]])
     from pigfem_assignment_amd import configs, _lib as L
     from pigfem_assignment_amd.api import HostProblem
@@ -41,46 +48,49 @@ def _rve(r, shift=0.0, N=24, r1=0.09):
                        materials=(configs.MATRIX, configs.INCLUSION), bcs=bcs, periodic=True)
 
 
+def _rich(t, h):
+    """Richardson-extrapolated central difference from tables at r +-h, +-2h"""
+    return (4.0 * (t[1] - t[-1]) / (2 * h) - (t[2] - t[-2]) / (4 * h)) / 3.0
+
+
 @pytest.mark.parametrize("which", [0, 1])
 def test_tables_match_central_differences(which):
-    r, r1, h = 0.265, 0.09, 1e-6
+    r, r1, h = R0, R1, 1e-6
     p0 = _rve(r, r1=r1)
-    pp = _rve(r + h, r1=r1) if which == 0 else _rve(r, r1=r1 + h)
-    pm = _rve(r - h, r1=r1) if which == 0 else _rve(r, r1=r1 - h)
-    assert p0.info.n_cut == pp.info.n_cut == pm.info.n_cut
-    assert np.array_equal(p0.cut(), pp.cut()) and np.array_equal(p0.cut(), pm.cut())
+    P = {m: (_rve(r + m * h, r1=r1) if which == 0 else _rve(r, r1=r1 + m * h)) for m in (1, -1, 2, -2)}
+    for q in P.values():
+        assert p0.info.n_cut == q.info.n_cut and np.array_equal(p0.cut(), q.cut())
     dgrad, divv, param, vel = sens.host_tables(p0)
     nn = p0.info.n_nodes
     # enrichment-node velocity (each node w.r.t. its own inclusion's radius);
     # the other inclusion's nodes stay put
-    Xp, Xm = pp.coords(), pm.coords()
-    fd_v = ((Xp[:2, nn:] - Xm[:2, nn:]) / (2 * h)).T
+    fd_v = _rich({m: q.coords()[:2, nn:] for m, q in P.items()}, h).T
     moving = p0.enrich_edges()[:, 2] == which
-    assert np.allclose(fd_v[moving], vel[moving, :2], rtol=1e-6, atol=1e-8)
+    assert np.allclose(fd_v[moving], vel[moving, :2], rtol=1e-8, atol=1e-10)
     assert np.max(np.abs(fd_v[~moving])) < 1e-9
     vp, w, go, g = _tables(p0)
-    _, wp, _, gp = _tables(pp)
-    _, wm, _, gm = _tables(pm)
+    T = {m: _tables(q) for m, q in P.items()}
     # points of the cut elements of the perturbed inclusion
     pts = np.concatenate([np.arange(vp[c], vp[c + 1]) for c in np.nonzero(param == which)[0]])
     others = np.setdiff1d(np.arange(len(w)), pts)
-    fd_w = (wp - wm) / (2 * h)
+    fd_w = _rich({m: t[1] for m, t in T.items()}, h)
     scale_w = np.max(np.abs(w[pts] * divv[pts]))
-    assert np.max(np.abs(fd_w[pts] - w[pts] * divv[pts])) <= 1e-6 * scale_w
+    err_w = np.max(np.abs(fd_w[pts] - w[pts] * divv[pts])) / scale_w
+    assert err_w <= 1e-8
     assert np.max(np.abs(fd_w[others])) <= 1e-9 * scale_w
     gidx = np.concatenate([np.arange(go[q], go[q] + (g.size if q + 1 == len(go) else go[q + 1]) - go[q])
                            for q in pts])
-    fd_g = (gp - gm) / (2 * h)
+    fd_g = _rich({m: t[3] for m, t in T.items()}, h)
     scale_g = np.max(np.abs(dgrad[gidx]))
     err = np.max(np.abs(fd_g[gidx] - dgrad[gidx])) / scale_g
     print("inclusion %d: max |FD(J W) - J W div v| / max = %.2e; max |FD(grad) - dgrad| / max = %.2e" % (
-        which, np.max(np.abs(fd_w[pts] - w[pts] * divv[pts])) / scale_w, err))
-    assert err <= 1e-5
+        which, err_w, err))
+    assert err <= 1e-8
     assert set(np.unique(param)) == {0, 1}
 
 
 def test_reference_parent_term_differs_and_is_not_translation_invariant():
-    r = 0.265
+    r = R0
     ex0 = sens.host_tables(_rve(r))[0]
     rf0 = sens.host_tables(_rve(r), as_reference=True)[0]
     ex1 = sens.host_tables(_rve(r, shift=1.0))[0]
@@ -92,7 +102,7 @@ def test_reference_parent_term_differs_and_is_not_translation_invariant():
     d0 = np.max(np.abs(rf0 - ex0)) / scale
     d1 = np.max(np.abs(rf1 - rf0)) / scale
     print("reference vs exact parent term: %.2e of max; change under translation: %.2e" % (d0, d1))
-    assert d0 > 1e-6 and d1 > 1e-6
+    assert d0 > 1e-7 and d1 > 1e-7
     assert d0 < 0.5
 
 
