@@ -255,10 +255,12 @@ PGF_API int pgf_set_shape_sensitivity(pgf_ctx* ctx, const double* dgrad, const d
 /* At the current nodal solution and macro strain eps_bar [S] (zero for a
  * problem without body load): rhs [n_free] = -(dP_int/dr - dF_body/dr) of
  * parameter slot `slot` (the direct method's right-hand side, K du/dr = rhs);
- * dsig [n_seg][S] = integral over each element segment of the explicit
- * stress derivative D dB u + sigma div v (the homogenised-stress sensitivity
- * is (integral of D B du/dr + dsig) / volume).  rhs may be NULL; n_seg may
- * be 0. */
+ * dsig [n_seg][S + 1] = integral over each element segment of the explicit
+ * stress derivative D dB u + sigma div v, then the derivative of the
+ * segment's volume, the sum of J*W div v (the discrete volume moves with r
+ * where a quadrature does not integrate the area exactly).  The
+ * homogenised-stress sensitivity is (integral of D B du/dr + dsig[:S]) /
+ * volume - sigma_bar dsig[S] / volume.  rhs may be NULL; n_seg may be 0. */
 PGF_API int pgf_shape_pseudo_load(pgf_ctx* ctx, const double* eps_bar, int32_t slot, double* rhs, int32_t n_seg,
                                   const int64_t* seg_elem, double* dsig);
 

@@ -12,6 +12,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpgfem.so")
+# compiled C++ caller of the C-ABI (pgfem.h / pgfem_host.h only), linked to the library
+DRIVER_SRC = os.path.join(CSRC, "driver", "pgf_driver.cpp")
+DRIVER = os.path.join(HERE, "pgf_driver")
 
 HOST_SRC = ["host/pgf_mesh.cpp", "host/pgf_dofs.cpp", "host/pgf_problem.cpp", "host/pgf_sens.cpp",
             "host/pgf_writer.cpp", "host/pgf_batch_host.cpp"]
@@ -53,6 +56,10 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lgomp"])
     os.replace(tmp, LIB)
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC, "-o", tmp,
+          "-L" + HERE, "-lpgfem", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
     return LIB
 
 

@@ -769,7 +769,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     HIP_DYNAMIC_SHARED(double, s_seg)
     constexpr int NU = GATHER_THREADS / SL;  // units per block
     __shared__ double s_P[NU * NR];
-    const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
+    const int64_t blk = a.gblk_lo + xcd_logical(blockIdx.x, gridDim.x);
     if (blk >= a.n_gblk) return;  // padding block of the XCD mapping
     const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
     const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
@@ -1050,24 +1050,33 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     }
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
-    const dim3 grid((unsigned)pgfk::xcd_blocks(c.n_gblk)), blk(pgfk::GATHER_THREADS);
-#define PGF_GATHER(SL, NR) hipLaunchKernelGGL((pgfk::k_gather<DIM, SL, NR, DO_K, DO_F, KT>), grid, blk, lds, c.stream, a)
-    if (c.gnode) {
-        switch (c.gslots) {
-            case 1: PGF_GATHER(1, DIM); break;
-            case 2: PGF_GATHER(2, DIM); break;
-            case 4: PGF_GATHER(4, DIM); break;
-            default: PGF_GATHER(8, DIM); break;
+    // regular-node blocks, then enrichment-node blocks (rows filled from the
+    // cut elements' K_e only): two launches of the same kernel
+    const int64_t ranges[2][2] = {{0, c.n_gblk_reg}, {c.n_gblk_reg, c.n_gblk}};
+    for (const auto& rg : ranges) {
+        if (rg[1] <= rg[0]) continue;
+        pgfk::AsmArgs ar = a;
+        ar.gblk_lo = rg[0];
+        ar.n_gblk = rg[1];
+        const dim3 grid((unsigned)pgfk::xcd_blocks(rg[1] - rg[0])), blk(pgfk::GATHER_THREADS);
+#define PGF_GATHER(SL, NR) hipLaunchKernelGGL((pgfk::k_gather<DIM, SL, NR, DO_K, DO_F, KT>), grid, blk, lds, c.stream, ar)
+        if (c.gnode) {
+            switch (c.gslots) {
+                case 1: PGF_GATHER(1, DIM); break;
+                case 2: PGF_GATHER(2, DIM); break;
+                case 4: PGF_GATHER(4, DIM); break;
+                default: PGF_GATHER(8, DIM); break;
+            }
+        } else {
+            switch (c.gslots) {
+                case 1: PGF_GATHER(1, 1); break;
+                case 2: PGF_GATHER(2, 1); break;
+                case 4: PGF_GATHER(4, 1); break;
+                default: PGF_GATHER(8, 1); break;
+            }
         }
-    } else {
-        switch (c.gslots) {
-            case 1: PGF_GATHER(1, 1); break;
-            case 2: PGF_GATHER(2, 1); break;
-            case 4: PGF_GATHER(4, 1); break;
-            default: PGF_GATHER(8, 1); break;
-        }
-    }
 #undef PGF_GATHER
+    }
     if (c.timing_events) (void)hipEventRecord(c.timing_events[2], c.stream);
 }
 

@@ -203,7 +203,9 @@ void build_gather_blocks(Ctx& c, int slots, bool node_rows) {
     int64_t seg_max = 0;
     for (int64_t n0 = 0; n0 < na;) {
         int64_t n1 = n0, rmin = INT64_MAX, rmax = -1, cnt = 0, inc_max = 0;
-        while (n1 < na && n1 - n0 < per_blk) {
+        // blocks do not straddle the last regular node: regular and enrichment
+        // nodes are gathered by separate launches
+        while (n1 < na && n1 - n0 < per_blk && !(n1 == c.n_nodes && n1 > n0)) {
             int64_t mn = rmin, mx = rmax, ct = cnt;
             // a periodic image owns no rows: the node holding its dofs first does
             const bool owns = c.h_image.empty() || !c.h_image[n1];
@@ -228,6 +230,8 @@ void build_gather_blocks(Ctx& c, int slots, bool node_rows) {
         bn.push_back(n1);
         n0 = n1;
     }
+    int64_t n_reg_blk = 0;
+    while (n_reg_blk + 1 < (int64_t)bn.size() && bn[n_reg_blk + 1] <= c.n_nodes) ++n_reg_blk;
     if ((int64_t)bn.size() - 1 > 0x7fffffff) throw pgf::Error(PGF_ELIMIT, "too many gather blocks");
     int64_t* node = dupload(bn.data(), (int64_t)bn.size());
     int64_t* seg = dupload(bs.data(), std::max<int64_t>((int64_t)bs.size(), 2));
@@ -237,6 +241,7 @@ void build_gather_blocks(Ctx& c, int slots, bool node_rows) {
     if (c.blk_rounds) HIPCHK(hipFree(c.blk_rounds));
     c.blk_node = node; c.blk_seg = seg; c.blk_rounds = rounds;
     c.n_gblk = (int64_t)bn.size() - 1;
+    c.n_gblk_reg = n_reg_blk;
     c.seg_max = seg_max;
     c.gslots = slots;
     c.gnode = node_rows;

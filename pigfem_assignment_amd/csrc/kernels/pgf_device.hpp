@@ -53,6 +53,7 @@ struct Ctx {
     int64_t *blk_node = nullptr, *blk_seg = nullptr;
     int32_t* blk_rounds = nullptr;
     int64_t n_gblk = 0, seg_max = 0;
+    int64_t n_gblk_reg = 0;         // gather blocks of the regular nodes (the rest: enrichment nodes)
     int gslots = 1;                 // gather pass: threads per row (or per node with gnode)
     bool gnode = false;             // gather pass: one thread serves all rows of a node
     int64_t n_vpt = 0, n_vgrad = 0;  // volumetric points of the cut elements and their gradient entries

@@ -96,7 +96,8 @@ struct AsmArgs {
     const int64_t* ke_ptr;
     double* pe;                // cut-element P_e scratch
     const int64_t* pe_ptr;
-    int64_t n_gblk;            // gather blocks
+    int64_t n_gblk;            // end of the gather blocks of this launch
+    int64_t gblk_lo;           // first gather block of this launch
     const int64_t* blk_node;   // [n_gblk+1] first node of each gather block
     const int64_t* blk_seg;    // [n_gblk][2] CSR segment of the block's free rows (empty: not staged)
     const int32_t* blk_rounds; // [n_gblk] incidence rounds: max incident elements of a block node / slots

@@ -90,7 +90,7 @@ struct SensArgs {
     const double* U;           // [dim][n_all]
     const double* eps_bar;     // [S]
     double* pe;                // [n_cut][MAX_NT * DIM] element pseudo-loads
-    double* ds;                // [n_cut][S] explicit stress derivative integrals
+    double* ds;                // [n_cut][S + 1] explicit stress derivative integrals, then sum of w div v
 };
 
 template <int DIM>
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(128) void k_sens_elem(SensArgs a) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n_cut) return;
     const int nt = a.ce_nt[c], M = nt * DIM;
-    double u[MAX_NT * DIM], g[MAX_NT * DIM], eb[S], ds[S];
+    double u[MAX_NT * DIM], g[MAX_NT * DIM], eb[S], ds[S], dvol = 0.0;
     for (int i = 0; i < S; ++i) {
         eb[i] = a.eps_bar[i];
         ds[i] = 0.0;
@@ -124,12 +124,14 @@ __global__ __launch_bounds__(128) void k_sens_elem(SensArgs a) {
             t[i] = dsig[i] + sig[i] * dv;
             ds[i] += w * t[i];
         }
+        dvol += w * dv;
         for (int s = 0; s < nt; ++s)
             for (int d = 0; d < DIM; ++d)
                 g[s * DIM + d] += w * (sens_bts<DIM>(gr + s * DIM, d, t) + sens_bts<DIM>(dg + s * DIM, d, sig));
     }
     for (int i = 0; i < M; ++i) a.pe[c * (MAX_NT * DIM) + i] = g[i];
-    for (int i = 0; i < S; ++i) a.ds[c * S + i] = ds[i];
+    for (int i = 0; i < S; ++i) a.ds[c * (S + 1) + i] = ds[i];
+    a.ds[c * (S + 1) + S] = dvol;
 }
 
 // rhs[dof] = -(sum over the node's incident cut elements of parameter `slot`,
@@ -206,7 +208,7 @@ PGF_API int pgf_set_shape_sensitivity(pgf_ctx* h, const double* dgrad, const dou
         c->sens_param = up(a->pr, c->n_cut);
         const int S = c->dim == 2 ? 3 : 6;
         SCHK(hipMalloc((void**)&c->sens_pe, sizeof(double) * std::max<int64_t>(c->n_cut, 1) * pgfk::MAX_NT * c->dim));
-        SCHK(hipMalloc((void**)&c->sens_ds, sizeof(double) * std::max<int64_t>(c->n_cut, 1) * S));
+        SCHK(hipMalloc((void**)&c->sens_ds, sizeof(double) * std::max<int64_t>(c->n_cut, 1) * (S + 1)));
         c->h_sens_param.assign(a->pr, a->pr + c->n_cut);
         return 0;
     }, &args);
@@ -255,22 +257,24 @@ PGF_API int pgf_shape_pseudo_load(pgf_ctx* h, const double* eps_bar, int32_t slo
             }
             SCHK(hipMemcpyAsync(a->rhs, drhs, sizeof(double) * c->n_free, hipMemcpyDeviceToHost, c->stream));
         }
-        std::vector<double> ds((size_t)std::max<int64_t>(c->n_cut, 1) * S);
+        std::vector<double> ds((size_t)std::max<int64_t>(c->n_cut, 1) * (S + 1));
         if (a->n_seg > 0 && c->n_cut > 0)
-            SCHK(hipMemcpyAsync(ds.data(), c->sens_ds, sizeof(double) * c->n_cut * S, hipMemcpyDeviceToHost, c->stream));
+            SCHK(hipMemcpyAsync(ds.data(), c->sens_ds, sizeof(double) * c->n_cut * (S + 1), hipMemcpyDeviceToHost,
+                                c->stream));
         SCHK(hipStreamSynchronize(c->stream));
         SCHK(hipFree(deb));
         SCHK(hipFree(drhs));
         if (a->n_seg > 0) {
-            // integrals of the explicit stress derivative per segment, cut elements in order
-            std::fill(a->dsig, a->dsig + (size_t)a->n_seg * S, 0.0);
+            // integrals of the explicit stress derivative and the volume
+            // derivative per segment, cut elements in order
+            std::fill(a->dsig, a->dsig + (size_t)a->n_seg * (S + 1), 0.0);
             for (int64_t ci = 0; ci < c->n_cut; ++ci) {
                 if (c->h_sens_param[ci] != a->slot) continue;
                 const int64_t e = c->h_cut_elem[ci];
                 const int64_t* it = std::upper_bound(a->seg, a->seg + a->n_seg + 1, e);
                 const int64_t k = (it - a->seg) - 1;
                 if (k < 0 || k >= a->n_seg) continue;
-                for (int i = 0; i < S; ++i) a->dsig[k * S + i] += ds[ci * S + i];
+                for (int i = 0; i <= S; ++i) a->dsig[k * (S + 1) + i] += ds[ci * (S + 1) + i];
             }
         }
         return 0;

@@ -71,13 +71,14 @@ def attach(ctx, dgrad, divv, param):
 
 def pseudo_load(ctx, eps_bar, slot, seg_elem=None):
     """(rhs [n_free] = -dR/dr_slot, explicit stress-derivative integrals
-    [n_seg][S]) at the context's current nodal solution"""
+    [n_seg][S] with the segment volume's derivative as column S) at the
+    context's current nodal solution"""
     lib = _lib()
     e = np.ascontiguousarray(eps_bar, np.float64)
     rhs = np.zeros(ctx.csr.n_free)
     seg = None if seg_elem is None else np.ascontiguousarray(seg_elem, np.int64)
     n_seg = 0 if seg is None else len(seg) - 1
-    dsig = np.zeros((max(n_seg, 1), len(e)))
+    dsig = np.zeros((max(n_seg, 1), len(e) + 1))
     ctx._chk(lib.pgf_shape_pseudo_load(ctx.h, L.ptr(e, C.c_double), int(slot), L.ptr(rhs, C.c_double), n_seg,
                                        None if seg is None else seg.ctypes.data_as(C.POINTER(C.c_int64)),
                                        L.ptr(dsig, C.c_double)))
@@ -88,12 +89,18 @@ def stiffness_sensitivities(ctx, X, seg_elem, sys_ptr, n_slots, cases, rtol=1e-1
     """dC_eff/dr [n_seg][n_slots][3][3] by the direct method: for every
     parameter slot, the pseudo-loads of the load cases (fluctuations X[k]) are
     solved together in one segmented batched PCG, and the homogenised-stress
-    derivative is the volume average of D B du/dr plus the explicit part."""
+    derivative is the volume average of D B du/dr plus the explicit part,
+    less sigma_bar times the relative volume derivative (the discrete volume,
+    a sum of J*W, moves with r where a quadrature is not exact for the area)."""
     n_seg = len(seg_elem) - 1
     S = len(cases[0])
     dC = np.zeros((n_seg, n_slots, S, len(cases)))
     vol = None
     info = []
+    sig_bar = []
+    for k, e in enumerate(cases):
+        ctx.set_solution_dofs(X[k])
+        sig_bar.append(ctx.volume_average(np.asarray(e, np.float64), seg_elem)[0])
     for j in range(n_slots):
         rhs, expl = [], []
         for k, e in enumerate(cases):
@@ -107,7 +114,7 @@ def stiffness_sensitivities(ctx, X, seg_elem, sys_ptr, n_slots, cases, rtol=1e-1
             ctx.set_solution_dofs(dX[k])
             sig, _, v = ctx.volume_average(np.zeros(S), seg_elem)
             vol = v
-            dC[:, j, :, k] = sig + expl[k] / v[:, None]
+            dC[:, j, :, k] = sig + (expl[k][:, :S] - sig_bar[k] * expl[k][:, S:]) / v[:, None]
         if log:
             log("# slot %d: batched sensitivity solves %.1f ms, iterations %d..%d" % (j, ms, its.min(), its.max()))
     return dC, info
