@@ -51,16 +51,22 @@ def gather_bytes(dim, nnz_rows, n_rows, n_all, n_elem):
 
 def tune_gather_layout(ctx, flags, reps=5, log=print):
     """Time the gather pass for every layout, 1, 2, 4 or 8 threads per unit
-    and a unit being one matrix row or all rows of a node (results are
-    bit-identical), and keep the fastest; untimed set-up, like an autotuner."""
+    and a unit being one matrix row or all rows of a node, and in 2D the
+    element-staged rows (results are bit-identical), and keep the fastest;
+    untimed set-up, like an autotuner."""
     ms = {}
-    for node_rows in (0, 1):
-        for slots in (1, 2, 4, 8):
-            ctx.gather_layout = (slots, node_rows)
-            ctx.assemble(flags)
-            ms["%s%d" % ("node" if node_rows else "row", slots)] = ctx.time_assemble_split(flags, reps=reps)[2]
+    layouts = [(slots, node_rows) for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
+    if ctx.prob.dim == 2:
+        layouts.append((1, 2))
+    names = {}
+    for slots, node_rows in layouts:
+        ctx.gather_layout = (slots, node_rows)
+        ctx.assemble(flags)
+        name = "staged1" if node_rows == 2 else "%s%d" % ("node" if node_rows else "row", slots)
+        names[name] = (slots, node_rows)
+        ms[name] = ctx.time_assemble_split(flags, reps=reps)[2]
     best = min(ms, key=ms.get)
-    ctx.gather_layout = (int(best[-1]), 1 if best.startswith("node") else 0)
+    ctx.gather_layout = names[best]
     log("# gather pass ms by layout (threads per row / per node): %s -> %s" % (
         ", ".join("%s: %.3f" % (k, v) for k, v in ms.items()), best))
     return best, ms
@@ -87,6 +93,7 @@ def main():
     ap.add_argument("--c3-rves", type=int, default=256)
     ap.add_argument("--c3-nmin", type=int, default=64)
     ap.add_argument("--c3-nmax", type=int, default=512)
+    ap.add_argument("--c3-slots", type=int, default=1, help="inclusion radii per RVE with direct sensitivities")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -486,42 +493,61 @@ def bench_c2(args):
 
 
 def bench_c3(args):
-    from pigfem_assignment_amd import batch, _lib as L
+    """Batch of periodic 2D RVEs (the reference's linear subscale model): one
+    batched assembly, the three macro-strain load cases of every RVE solved by
+    one segmented multi-RHS PCG to 1e-8, C_eff from the volume-averaged
+    stress, then the direct sensitivities dC_eff/dr of the first --c3-slots
+    inclusion radii of every RVE (each slot: pseudo-loads of 3 cases x all
+    RVEs in one more batched solve with the same K)."""
+    from pigfem_assignment_amd import batch, sens, _lib as L
     from pigfem_assignment_amd.api import DeviceContext
     rng = np.random.RandomState(20261202)
     sizes = rng.randint(args.c3_nmin, args.c3_nmax + 1, size=args.c3_rves)
     t0 = time.time()
-    parts = [batch.rve_problem(int(N), 1000 + k)[0] for k, N in enumerate(sizes)]
-    B = batch.BatchProblem(parts)
-    print("# C3 set-up: %d RVEs, %d elements (%d cut), %d dofs, host %.1f s" % (
-        len(parts), B.n_elem, B.n_cut, B.n_free, time.time() - t0), flush=True)
+    parts, incls = zip(*[batch.periodic_rve_problem(int(N), 1000 + k) for k, N in enumerate(sizes)])
+    B = batch.BatchProblem(list(parts))
+    setup_s = time.time() - t0
+    print("# C3 set-up: %d periodic RVEs, %d elements (%d cut), %d dofs, host %.1f s" % (
+        len(parts), B.n_elem, B.n_cut, B.n_free, setup_s), flush=True)
     ctx = DeviceContext(B)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS
     slots, _ = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags)
     ms_asm = ctx.time_assemble(flags, reps=args.steps)
-    UP = B.case_values()
-    RHS = ctx.rhs_prescribed(UP)
-    X, its, rel, ms_solve, tune = ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-8)
-    R = ctx.reactions(X, UP)
-    Ce = batch.effective_stiffness(B, R)
+    F = batch.macro_strain_rhs(ctx, B)
+    ctx.assemble(L.PGF_ASM_JAC)
+    X, its, rel, ms_solve, tune = ctx.batch_pcg(B.sys_ptr, F, rtol=1e-8)
+    Ce = batch.homogenized_stiffness(ctx, B, X, B.elem_ptr)
     print("# C3 assembly %.2f ms (%.3e elem/s); batched PCG 3 cases x %d RVEs: %.1f ms, iterations %d..%d, "
           "tuned (threads/row, rows/tile) %s" % (ms_asm, B.n_elem / (ms_asm * 1e-3), len(parts), ms_solve,
                                                  its.min(), its.max(), tune), flush=True)
+    extra = {"batched_solve_ms": ms_solve, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
+             "iterations_total": int(its.sum()), "autotuned": list(tune), "gather_layout": slots,
+             "host_setup_s": setup_s, "C_eff_mean": Ce.mean(axis=0).tolist()}
+    if args.c3_slots > 0:
+        t1 = time.time()
+        sens.attach(ctx, *sens.batch_tables(list(parts)))
+        dC, info = sens.stiffness_sensitivities(ctx, X, B.elem_ptr, B.sys_ptr, args.c3_slots, batch.VOIGT_CASES,
+                                                rtol=1e-8, log=lambda m: print(m, flush=True))
+        wall = time.time() - t1
+        extra.update({"sensitivity_slots": args.c3_slots,
+                      "sensitivity_solve_ms": [float(i[2]) for i in info],
+                      "sensitivity_iterations_max": [int(i[0].max()) for i in info],
+                      "sensitivity_wall_s": wall,
+                      "dC_dr0_mean": dC[:, 0].mean(axis=0).tolist()})
+        print("# C3 sensitivities: %d slot(s) x 3 cases x %d RVEs, batched solves %s ms, %.1f s wall" % (
+            args.c3_slots, len(parts), ["%.0f" % i[2] for i in info], wall), flush=True)
     line = {
         "metric": "element assemblies/s; time-to-1e-8 residual per load step, learned vs plain",
         "value": B.n_elem / (ms_asm * 1e-3), "unit": "element assemblies/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_asm, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "%d 2D RVEs N~U[%d,%d] QUAD4, 20-40%% inclusions, 3 homogenisation load cases, "
-                               "one batched assembly + segmented multi-RHS PCG to 1e-8" % (
-                                   len(parts), args.c3_nmin, args.c3_nmax),
+        "config": {"workload": "%d periodic 2D RVEs N~U[%d,%d] QUAD4, 20-40%% inclusions, 3 macro-strain load "
+                               "cases + radius sensitivities of %d slot(s), one batched assembly + segmented "
+                               "multi-RHS PCG to 1e-8" % (len(parts), args.c3_nmin, args.c3_nmax, args.c3_slots),
                    "elements": B.n_elem, "dofs": B.n_free},
-        "roofline": None, "cpu_baseline": None,
-        "extra": {"batched_solve_ms": ms_solve, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
-                  "iterations_total": int(its.sum()), "autotuned": list(tune), "gather_layout": slots,
-                  "C_eff_mean": Ce.mean(axis=0).tolist()},
+        "roofline": None, "cpu_baseline": None, "extra": extra,
     }
     print(json.dumps(line))
 

@@ -535,8 +535,9 @@ __global__ __launch_bounds__(cut_threads<DIM>()) void k_cut_elem(AsmArgs a) {
 // sub-element points, then cohesive points) is one small dense product
 //     K_e = A^T C,
 // with the rows of A and C stacked over the element's integration points:
-//   * volumetric point q (6 rows, Voigt s):  A = B_q,   C = w_q D_q B_q
-//     (B^T D B of SmallStrainKFast, src/Utilities.cpp:196-296);
+//   * volumetric point q (7 rows): with the isotropic D = lam m m^T + mu D0,
+//     A = (m^T B_q; B_q) and C = diag(w lam, 2 w mu (x3), w mu (x3)) A, so
+//     A^T C = w B^T D B (SmallStrainKFast, src/Utilities.cpp:196-296);
 //   * cohesive point p (3 rows, jump component d):  A = L_p,  C = w_p T_p L_p
 //     with L_p (u) = sum_n N_n (u_upper,n - u_lower,n) and T_p = R^T D_c R
 //     (KernelCohesive / cohesiveKFast, src/AssemblerSmallStrainNonlinear.cpp:73-91,
@@ -563,6 +564,7 @@ struct MfWave {
     double g[MF_NV * MF_MM];      // slot gradients per point
     double w[MF_NV];
     double Dabc[MF_NV * 3];       // isotropic D: a (normal), b (off-diagonal), c (shear)
+    double wl[MF_NV], wm[MF_NV];  // w lam and w mu of every point (lam = b, mu = c)
     double sig[MF_NV * 6];
     double u[MF_MM];
     double T[MAX_CPT * 9];        // w_p R^T D_c R
@@ -572,47 +574,26 @@ struct MfWave {
     int nv, nc, nt;
 };
 
-// B_q(s, (a, da)) of a volumetric point from its slot gradient ga
-__device__ inline double b_entry(const double* ga, int s, int da) {
-    if (s < 3) return s == da ? ga[da] : 0.0;
-    if (s == 3) return da == 1 ? ga[2] : (da == 2 ? ga[1] : 0.0);
-    if (s == 4) return da == 0 ? ga[2] : (da == 2 ? ga[0] : 0.0);
-    return da == 0 ? ga[1] : (da == 1 ? ga[0] : 0.0);
-}
-// (D B_q)(s, (b, db)) for the isotropic D (a, b, c)
-__device__ inline double db_entry(const double* gb, int s, int db, double a, double b, double c) {
-    if (s < 3) return (s == db ? a : b) * gb[db];
-    return c * b_entry(gb, s, db);
-}
-
-// A(kk, i) and C(kk, j) of the stacked point rows (zero beyond the element's rows)
-__device__ inline double a_val(const MfWave& W, int kk, int i, int M) {
-    if (i >= M) return 0.0;
-    const int a = i / 3, da = i - 3 * a;
-    const int kv = 6 * W.nv;
-    if (kk < kv) {
-        const int q = kk / 6, s = kk - 6 * q;
-        return b_entry(&W.g[q * MF_MM + a * 3], s, da);
-    }
-    const int kc = kk - kv;
-    if (kc >= 3 * W.nc) return 0.0;
-    const int p = kc / 3, d = kc - 3 * p;
-    return d == da ? W.cs[p * MAX_NT + a] : 0.0;
-}
-__device__ inline double c_val(const MfWave& W, int kk, int j, int M) {
-    if (j >= M) return 0.0;
-    const int b = j / 3, db = j - 3 * b;
-    const int kv = 6 * W.nv;
-    if (kk < kv) {
-        const int q = kk / 6, s = kk - 6 * q;
-        return W.w[q] * db_entry(&W.g[q * MF_MM + b * 3], s, db, W.Dabc[3 * q], W.Dabc[3 * q + 1], W.Dabc[3 * q + 2]);
-    }
-    const int kc = kk - kv;
-    if (kc >= 3 * W.nc) return 0.0;
-    const int p = kc / 3, d = kc - 3 * p;
-    return W.T[p * 9 + d * 3 + db] * W.cs[p * MAX_NT + b];
+// Operand rows of a volumetric point q for the isotropic D = lam m m^T + mu D0,
+// D0 = diag(2, 2, 2, 1, 1, 1): B^T D B = lam (m^T B)^T (m^T B) + mu B^T D0 B,
+// so with the seven rows
+//   s = 0: (m^T B)(a, da) = g_a[da]                      weight w lam
+//   s = 1..3: normal Voigt row c = s - 1 of B            weight 2 w mu
+//   s = 4..6: shear Voigt rows yz, xz, xy of B           weight w mu
+// C is the weight times A, and an entry of A is one gradient component or 0.
+// A(kk, (a, da)) for row s: the gradient component it selects, or -1 for 0.
+__device__ inline int vrow_comp(int s, int da) {
+    if (s == 0) return da;
+    if (s <= 3) return s - 1 == da ? da : -1;
+    if (s == 4) return da == 0 ? -1 : 3 - da;  // yz: u_y <- g_z, u_z <- g_y
+    if (s == 5) return da == 1 ? -1 : 2 - da;  // xz: u_x <- g_z, u_z <- g_x
+    return da == 2 ? -1 : 1 - da;              // xy: u_x <- g_y, u_y <- g_x
 }
 
+// K_e = A^T C over the stacked rows: 7 per volumetric point, then 3 per
+// cohesive point (A = d-th jump row, C = T_p rows times the signed shape
+// functions).  Each lane evaluates, per k-step, the A entries of its column
+// in every tile (NT values) and scales them into C for volumetric rows.
 template <int NT>
 __device__ inline void mfma_ke(const MfWave& W, int M, int lane, double* ke) {
     v4d acc[NT][NT];
@@ -620,15 +601,43 @@ __device__ inline void mfma_ke(const MfWave& W, int M, int lane, double* ke) {
     for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
         for (int tj = 0; tj < NT; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
-    const int KD = 6 * W.nv + 3 * W.nc;
+    const int KV = 7 * W.nv, KD = KV + 3 * W.nc;
     const int r16 = lane & 15, k4 = lane >> 4;
+    // this lane's column of every tile: slot a, component da
+    int ga[NT], da[NT];
+    bool ok[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = 16 * t + r16;
+        ok[t] = i < M;
+        const int a = i / 3;
+        da[t] = i - 3 * a;
+        ga[t] = a * 3;
+    }
     for (int k0 = 0; k0 < KD; k0 += 4) {
         const int kk = k0 + k4;
         double av[NT], cv[NT];
+        if (kk < KV) {
+            const int q = kk / 7, s = kk - 7 * q;
+            const double wt = s == 0 ? W.wl[q] : (s <= 3 ? 2.0 * W.wm[q] : W.wm[q]);
+            const double* gq = &W.g[q * MF_MM];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            av[t] = kk < KD ? a_val(W, kk, 16 * t + r16, M) : 0.0;
-            cv[t] = kk < KD ? c_val(W, kk, 16 * t + r16, M) : 0.0;
+            for (int t = 0; t < NT; ++t) {
+                const int cmp = vrow_comp(s, da[t]);
+                av[t] = ok[t] && cmp >= 0 ? gq[ga[t] + cmp] : 0.0;
+                cv[t] = wt * av[t];
+            }
+        } else if (kk < KD) {
+            const int kc = kk - KV, p = kc / 3, d = kc - 3 * p;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const double sa = ok[t] ? W.cs[p * MAX_NT + ga[t] / 3] : 0.0;
+                av[t] = d == da[t] ? sa : 0.0;
+                cv[t] = W.T[p * 9 + d * 3 + da[t]] * sa;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) av[t] = cv[t] = 0.0;
         }
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti)
@@ -647,8 +656,12 @@ __device__ inline void mfma_ke(const MfWave& W, int M, int lane, double* ke) {
             }
 }
 
+// At least four waves per SIMD: the compiler then keeps the 3 x 3 tiles of
+// accumulators in ordinary VGPRs (128 in all, no spills) instead of 131 VGPRs
+// plus 72 accumulation registers at two waves per SIMD; the staging loads of
+// one element overlap the products of the others.
 template <bool DO_K, bool DO_F>
-__global__ __launch_bounds__(MF_WAVES * WAVE) void k_cut_elem_mfma(AsmArgs a) {
+__global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a) {
     __shared__ MfWave s_w[MF_WAVES];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
@@ -669,11 +682,14 @@ __global__ __launch_bounds__(MF_WAVES * WAVE) void k_cut_elem_mfma(AsmArgs a) {
         W.g[q * MF_MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
     }
     if (lane < nv) {
-        W.w[lane] = a.vpt_w[v0 + lane];
+        const double w = a.vpt_w[v0 + lane];
+        W.w[lane] = w;
         const double* D = a.Dmat + a.vpt_mat[v0 + lane] * 36;
         W.Dabc[3 * lane] = D[0];
         W.Dabc[3 * lane + 1] = D[1];
         W.Dabc[3 * lane + 2] = D[35];
+        W.wl[lane] = w * D[1];
+        W.wm[lane] = w * D[35];
     }
     for (int k = lane; k < nc * MAX_NT; k += WAVE) W.cs[k] = 0.0;
     __syncthreads();
@@ -750,6 +766,162 @@ __global__ __launch_bounds__(MF_WAVES * WAVE) void k_cut_elem_mfma(AsmArgs a) {
 }
 
 
+// Gather scheme, pass 2, element-staged variant (2D).  A block of ES_THREADS
+// threads owns a run of consecutive nodes, one thread per matrix row.  The
+// uncut elements incident to the block's nodes (a host list per block, in
+// element order) are evaluated once per block first: per Gauss point the
+// inverse Jacobian, J*W and (with the internal force) the stress, kept in
+// LDS.  Each row then forms its entries from those tables, so the element
+// geometry is computed once per block instead of once per row (eight times
+// per QUAD4).  The arithmetic is that of qp_geometry and of k_gather with one
+// row and one slot per thread, in the same order: the bits are the same.
+template <bool DO_K, bool DO_F, typename KT>
+__global__ __launch_bounds__(ES_THREADS) void k_gather_es(AsmArgs a) {
+    HIP_DYNAMIC_SHARED(double, s_dyn)
+    const int64_t blk = a.gblk_lo + xcd_logical(blockIdx.x, gridDim.x);
+    if (blk >= a.n_gblk) return;  // padding block of the XCD mapping
+    double* s_seg = s_dyn;
+    double* s_geo = s_dyn + a.es_seg_cap;
+    const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
+    const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
+    const bool use_lds = DO_K && s1 > s0;
+    const int t = threadIdx.x;
+    const int64_t n = n0 + t / 2;
+    const int da = t % 2;
+    const bool active = n < n1 && !(a.image && a.image[n]);
+    const int r = active ? a.dof[da * a.n_all + n] : 0;
+    const bool own = active && (r < a.n_free || a.all_rows);
+    const bool in_lds = use_lds && own && r < a.n_free;
+    const int64_t base = own ? a.rowptr[r] : 0;
+    if (use_lds)
+        for (int64_t j = t; j < s1 - s0; j += ES_THREADS) s_seg[j] = 0.0;
+    if (DO_K && own && !in_lds)
+        for (int32_t j = a.rowptr[r]; j < a.rowptr[r + 1]; ++j) {
+            if (sizeof(KT) == 8) a.K[j] = 0.0;
+            else a.K32[j] = 0.0f;
+        }
+    // ---- phase A: the block's uncut elements, once each ----
+    const int64_t e0 = a.blk_el_ptr[blk], ne = a.blk_el_ptr[blk + 1] - e0;
+    const double r0 = 0.57735026918962576450914878050196;
+    for (int64_t le = t; le < ne; le += ES_THREADS) {
+        const int e = a.blk_el[e0 + le];
+        const int4 cn = a.el_conn4[e];
+        const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
+        double X[4][2], u[8];
+        for (int q = 0; q < 4; ++q) {
+            const double2 xv = ((const double2*)a.Xq)[nd[q]];
+            X[q][0] = xv.x;
+            X[q][1] = xv.y;
+            if (DO_F)
+                for (int d = 0; d < 2; ++d) u[q * 2 + d] = a.U[d * a.n_all + nd[q]];
+        }
+        const Iso D = iso_of<2>(a.Dmat + a.el_mat[e] * 36);
+        double* geo = &s_geo[le * ES_GEO];
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            const double rr = (q % 2 == 0) ? -r0 : r0, ss = (q / 2 == 0) ? -r0 : r0;
+            const double dr[4][2] = {{-0.25 * (1.0 - ss), -0.25 * (1.0 - rr)},
+                                     {0.25 * (1.0 - ss), -0.25 * (1.0 + rr)},
+                                     {0.25 * (1.0 + ss), 0.25 * (1.0 + rr)},
+                                     {-0.25 * (1.0 + ss), 0.25 * (1.0 - rr)}};
+            double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+            for (int k = 0; k < 4; ++k) {
+                J00 += dr[k][0] * X[k][0]; J01 += dr[k][0] * X[k][1];
+                J10 += dr[k][1] * X[k][0]; J11 += dr[k][1] * X[k][1];
+            }
+            const double det = J00 * J11 - J01 * J10;
+            const double rd = 1.0 / det;
+            const double i00 = J11 * rd, i01 = -J01 * rd, i10 = -J10 * rd, i11 = J00 * rd;
+            geo[q * 8 + 0] = i00;
+            geo[q * 8 + 1] = i01;
+            geo[q * 8 + 2] = i10;
+            geo[q * 8 + 3] = i11;
+            geo[q * 8 + 4] = det * 1.0;
+            if (DO_F) {
+                double g[8], eps[3], sig[3];
+                for (int k = 0; k < 4; ++k) {
+                    g[k * 2 + 0] = i00 * dr[k][0] + i01 * dr[k][1];
+                    g[k * 2 + 1] = i10 * dr[k][0] + i11 * dr[k][1];
+                }
+                strain_of<2>(4, g, u, eps);
+                stress_of<2>(D, eps, sig);
+                geo[q * 8 + 5] = sig[0];
+                geo[q * 8 + 6] = sig[1];
+                geo[q * 8 + 7] = sig[2];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- phase B: each row over its incident elements, in element order ----
+    double Ptot = 0.0;
+    const int64_t kbeg = own ? a.inc_ptr[n] : 0, kend = own ? a.inc_ptr[n + 1] : 0;
+    for (int64_t k = kbeg; k < kend; ++k) {
+        const int2 rec = a.inc_rec[k];
+        const int la = rec.y & 0xff;
+        const int c = (rec.y & 0x100) ? rec.x : -1;
+        auto add = [&](int64_t rel, double v) {
+            if (in_lds) s_seg[base + rel - s0] += v;
+            else kadd<KT>(a, base + rel, v);
+        };
+        if (c < 0) {
+            const double* geo = &s_geo[(int64_t)a.inc_loc[k] * ES_GEO];
+            const Iso D = iso_of<2>(a.Dmat + (rec.y >> 16) * 36);
+            double Kr[8], Pc = 0.0;
+            for (int j = 0; j < 8; ++j) Kr[j] = 0.0;
+#pragma unroll 1
+            for (int q = 0; q < 4; ++q) {
+                const double rr = (q % 2 == 0) ? -r0 : r0, ss = (q / 2 == 0) ? -r0 : r0;
+                const double dr[4][2] = {{-0.25 * (1.0 - ss), -0.25 * (1.0 - rr)},
+                                         {0.25 * (1.0 - ss), -0.25 * (1.0 + rr)},
+                                         {0.25 * (1.0 + ss), 0.25 * (1.0 + rr)},
+                                         {-0.25 * (1.0 + ss), 0.25 * (1.0 - rr)}};
+                const double i00 = geo[q * 8 + 0], i01 = geo[q * 8 + 1], i10 = geo[q * 8 + 2], i11 = geo[q * 8 + 3];
+                const double w = geo[q * 8 + 4];
+                double g[8];
+                for (int j = 0; j < 4; ++j) {
+                    g[j * 2 + 0] = i00 * dr[j][0] + i01 * dr[j][1];
+                    g[j * 2 + 1] = i10 * dr[j][0] + i11 * dr[j][1];
+                }
+                double gr[2];
+                for (int i = 0; i < 2; ++i)
+                    gr[i] = la == 0 ? g[i] : la == 1 ? g[2 + i] : la == 2 ? g[4 + i] : g[6 + i];
+                if (DO_F) {
+                    const double sig[3] = {geo[q * 8 + 5], geo[q * 8 + 6], geo[q * 8 + 7]};
+                    Pc += w * force_entry<2>(gr, da, sig);
+                }
+                if (DO_K) {
+                    double parts[3];
+                    k_parts<2>(gr, da, D, parts);
+                    for (int bn = 0; bn < 4; ++bn)
+                        for (int db = 0; db < 2; ++db) Kr[bn * 2 + db] += w * k_entry<2>(parts, &g[bn * 2], db);
+                }
+            }
+            if (DO_F) Ptot += Pc;
+            if (DO_K) {
+                const uint32_t* rw = (const uint32_t*)(a.inc_rel + k * 8);
+                const uint32_t wv[2] = {rw[0], rw[1]};
+                for (int j = 0; j < 8; ++j) add((int)((wv[j / 4] >> (8 * (j % 4))) & 0xffu), Kr[j]);
+            }
+        } else {
+            if (DO_F) Ptot += a.pe[a.pe_ptr[c] + la * 2 + da];
+            if (DO_K) {
+                const int nt = a.ce_nt[c], M = nt * 2;
+                const uint8_t* rl = a.ce_rel + a.ce_rel_ptr[c] + (int64_t)la * M;
+                const double* kb = a.ke + a.ke_ptr[c] + (int64_t)(la * 2 + da) * M;
+                for (int j = 0; j < M; ++j) add(rl[j], kb[j]);
+            }
+        }
+    }
+    if (DO_F && own) a.P[r] = Ptot;
+    if (use_lds) {
+        __syncthreads();
+        for (int64_t j = t; j < s1 - s0; j += ES_THREADS) {
+            if (sizeof(KT) == 8) a.K[s0 + j] = s_seg[j];
+            else a.K32[s0 + j] = (float)s_seg[j];
+        }
+    }
+}
+
 // Gather scheme, pass 2.  A block owns a run of consecutive nodes.  Its
 // threads are (unit, slot) pairs: a unit is one matrix row (NR = 1) or all DIM
 // rows of one node (NR = DIM, the element geometry and the nodal data then
@@ -818,16 +990,26 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         for (int h = 0; h < NR; ++h) Pc[h] = 0.0;
         int c = -1, la = 0;
         if (valid) {
-            const int e = a.inc_elem[k];
-            la = a.inc_slot[k];
-            c = a.cut_index[e];
+            const int2 rec = a.inc_rec[k];
+            la = rec.y & 0xff;
+            c = (rec.y & 0x100) ? rec.x : -1;
             if (c < 0) {
                 double X[4][DIM];
+                const int4 cn = a.el_conn4[rec.x];
+                nd[0] = cn.x; nd[1] = cn.y; nd[2] = cn.z; nd[3] = cn.w;
                 for (int q = 0; q < 4; ++q) {
-                    nd[q] = a.el_conn[(int64_t)q * a.n_elem + e];
-                    for (int i = 0; i < DIM; ++i) X[q][i] = a.X[i * a.n_all + nd[q]];
+                    if (DIM == 2) {
+                        const double2 xv = ((const double2*)a.Xq)[nd[q]];
+                        X[q][0] = xv.x;
+                        X[q][1] = xv.y;
+                    } else {
+                        const double4 xv = ((const double4*)a.Xq)[nd[q]];
+                        X[q][0] = xv.x;
+                        X[q][1] = xv.y;
+                        X[q][DIM - 1] = xv.z;
+                    }
                 }
-                const int mat = a.el_mat[e];
+                const int mat = rec.y >> 16;
                 const Iso D = iso_of<DIM>(a.Dmat + mat * 36);
                 double u[4 * DIM];
                 if (DO_F)
@@ -1058,6 +1240,12 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
         pgfk::AsmArgs ar = a;
         ar.gblk_lo = rg[0];
         ar.n_gblk = rg[1];
+        if (DIM == 2 && c.ges) {
+            const size_t lds_es = sizeof(double) * (size_t)(c.seg_max + c.es_geo_max * pgfk::ES_GEO);
+            hipLaunchKernelGGL((pgfk::k_gather_es<DO_K, DO_F, KT>), dim3((unsigned)pgfk::xcd_blocks(rg[1] - rg[0])),
+                               dim3(pgfk::ES_THREADS), lds_es, c.stream, ar);
+            continue;
+        }
         const dim3 grid((unsigned)pgfk::xcd_blocks(rg[1] - rg[0])), blk(pgfk::GATHER_THREADS);
 #define PGF_GATHER(SL, NR) hipLaunchKernelGGL((pgfk::k_gather<DIM, SL, NR, DO_K, DO_F, KT>), grid, blk, lds, c.stream, ar)
         if (c.gnode) {

@@ -141,9 +141,9 @@ def test_3d_cohesive_stiffness(law, scheme):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_gather_layouts_bit_identical(dim):
     """Every layout of the gather pass (1, 2, 4, 8 threads per unit; a unit one
-    matrix row or all rows of a node) sums each entry's element contributions
-    in element order: K, P and the cohesive state are bit-identical across
-    layouts, and match the oracle."""
+    matrix row or all rows of a node; in 2D also the element-staged rows) sums
+    each entry's element contributions in element order: K, P and the cohesive
+    state are bit-identical across layouts, and match the oracle."""
     p, o = _c0(60) if dim == 2 else _c2small(configs.BILINEAR_C2)
     ctx = DeviceContext(p)
     rng = np.random.RandomState(5)
@@ -153,8 +153,11 @@ def test_gather_layouts_bit_identical(dim):
     vals, Po, _ = o.assemble(dt=0.05)
     rp = ctx.pattern()[0]
     ref = None
-    for node_rows in (0, 1):
-        for slots in (1, 2, 4, 8):
+    layouts = [(slots, node_rows) for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
+    if dim == 2:
+        layouts.append((1, 2))  # element-staged rows
+    for slots, node_rows in layouts:
+        if True:
             ctx.gather_layout = (slots, node_rows)
             assert ctx.gather_layout == (slots, node_rows)
             for flags in (L.PGF_ASM_JAC | L.PGF_ASM_FUNC | L.PGF_ASM_ALL_ROWS, L.PGF_ASM_JAC, L.PGF_ASM_FUNC):
@@ -172,7 +175,10 @@ def test_gather_layouts_bit_identical(dim):
     with pytest.raises(RuntimeError):
         ctx.gather_layout = (3, 0)
     with pytest.raises(RuntimeError):
-        ctx.gather_layout = (1, 2)
+        ctx.gather_layout = (1, 3)
+    if dim == 3:
+        with pytest.raises(RuntimeError):
+            ctx.gather_layout = (1, 2)  # element staging is 2D only
 
 
 @pytest.mark.parametrize("precision", [64, 32])
