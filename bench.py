@@ -51,18 +51,15 @@ def gather_bytes(dim, nnz_rows, n_rows, n_all, n_elem):
 
 def tune_gather_layout(ctx, flags, reps=5, log=print):
     """Time the gather pass for every layout, 1, 2, 4 or 8 threads per unit
-    and a unit being one matrix row or all rows of a node, and in 2D the
-    element-staged rows (results are bit-identical), and keep the fastest;
-    untimed set-up, like an autotuner."""
+    and a unit being one matrix row or all rows of a node (results are
+    bit-identical), and keep the fastest; untimed set-up, like an autotuner."""
     ms = {}
     layouts = [(slots, node_rows) for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
-    if ctx.prob.dim == 2:
-        layouts.append((1, 2))
     names = {}
     for slots, node_rows in layouts:
         ctx.gather_layout = (slots, node_rows)
         ctx.assemble(flags)
-        name = "staged1" if node_rows == 2 else "%s%d" % ("node" if node_rows else "row", slots)
+        name = "%s%d" % ("node" if node_rows else "row", slots)
         names[name] = (slots, node_rows)
         ms[name] = ctx.time_assemble_split(flags, reps=reps)[2]
     best = min(ms, key=ms.get)
@@ -427,14 +424,15 @@ def bench_c2(args):
     gb_cut = bytes_cut / (ms_cut * 1e-3) / 1e9
     print("# C2 assembly K+f FP64: %.2f ms (cut pass %.2f ms, %.0f GB/s; gather pass %.2f ms, %.0f GB/s), "
           "%.3e elem/s" % (ms, ms_cut, gb_cut, ms_gather, gb_gather, value), flush=True)
+    # the 3D cut pass is the matrix-core kernel k_cut_elem_mfma
     if ms_cut >= ms_gather:
-        dom = ("k_cut_elem", gb_cut, bytes_cut, ms_cut)
+        dom = ("k_cut_elem_mfma", gb_cut, bytes_cut, ms_cut)
         other = {"kernel": "k_gather", "achieved": gb_gather, "frac": gb_gather / HBM_PEAK_GBS, "ms": ms_gather,
-                 "algorithmic_bytes": bytes_gather}
+                 "algorithmic_bytes": bytes_gather, "traffic": measured_traffic("c2", "k_gather")}
     else:
         dom = ("k_gather", gb_gather, bytes_gather, ms_gather)
-        other = {"kernel": "k_cut_elem", "achieved": gb_cut, "frac": gb_cut / HBM_PEAK_GBS, "ms": ms_cut,
-                 "algorithmic_bytes": bytes_cut}
+        other = {"kernel": "k_cut_elem_mfma", "achieved": gb_cut, "frac": gb_cut / HBM_PEAK_GBS, "ms": ms_cut,
+                 "algorithmic_bytes": bytes_cut, "traffic": measured_traffic("c2", "k_cut_elem_mfma")}
     extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather, "cut_pass_bytes": bytes_cut,
              "gather_pass_bytes": bytes_gather, "gather_layout": slots, "gather_pass_ms_by_layout": slot_ms,
              "roofline_other_pass": other, "host_setup_s": setup_s, "cohesive_points": info.n_cpt}

@@ -278,9 +278,7 @@ PGF_API int pgf_time_spmv(pgf_ctx* ctx, int32_t precision, int32_t reps, double*
 /* ---- tuning of the row-gather assembly (no counterpart in the reference:
  * the reference assembles with MatSetValue, src/Assembler.cpp:811-832) ----
  * slots: threads that share one unit's incident elements, 1, 2, 4 or 8;
- * node_rows: 0 = a unit is one matrix row, 1 = a unit is all rows of a node,
- * 2 = one row per thread with the block's uncut elements evaluated once into
- * LDS first (2D only, slots must be 1).
+ * node_rows: 0 = a unit is one matrix row, 1 = a unit is all rows of a node.
  * Default: 1 slot, one row per unit in 2D; 2 slots in 3D.  Rebuilds the
  * node-block partition; the assembled values do not change (bit for bit). */
 PGF_API int pgf_set_gather_layout(pgf_ctx* ctx, int32_t slots, int32_t node_rows);

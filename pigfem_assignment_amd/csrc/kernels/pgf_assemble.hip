@@ -766,162 +766,6 @@ __global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a)
 }
 
 
-// Gather scheme, pass 2, element-staged variant (2D).  A block of ES_THREADS
-// threads owns a run of consecutive nodes, one thread per matrix row.  The
-// uncut elements incident to the block's nodes (a host list per block, in
-// element order) are evaluated once per block first: per Gauss point the
-// inverse Jacobian, J*W and (with the internal force) the stress, kept in
-// LDS.  Each row then forms its entries from those tables, so the element
-// geometry is computed once per block instead of once per row (eight times
-// per QUAD4).  The arithmetic is that of qp_geometry and of k_gather with one
-// row and one slot per thread, in the same order: the bits are the same.
-template <bool DO_K, bool DO_F, typename KT>
-__global__ __launch_bounds__(ES_THREADS) void k_gather_es(AsmArgs a) {
-    HIP_DYNAMIC_SHARED(double, s_dyn)
-    const int64_t blk = a.gblk_lo + xcd_logical(blockIdx.x, gridDim.x);
-    if (blk >= a.n_gblk) return;  // padding block of the XCD mapping
-    double* s_seg = s_dyn;
-    double* s_geo = s_dyn + a.es_seg_cap;
-    const int64_t n0 = a.blk_node[blk], n1 = a.blk_node[blk + 1];
-    const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
-    const bool use_lds = DO_K && s1 > s0;
-    const int t = threadIdx.x;
-    const int64_t n = n0 + t / 2;
-    const int da = t % 2;
-    const bool active = n < n1 && !(a.image && a.image[n]);
-    const int r = active ? a.dof[da * a.n_all + n] : 0;
-    const bool own = active && (r < a.n_free || a.all_rows);
-    const bool in_lds = use_lds && own && r < a.n_free;
-    const int64_t base = own ? a.rowptr[r] : 0;
-    if (use_lds)
-        for (int64_t j = t; j < s1 - s0; j += ES_THREADS) s_seg[j] = 0.0;
-    if (DO_K && own && !in_lds)
-        for (int32_t j = a.rowptr[r]; j < a.rowptr[r + 1]; ++j) {
-            if (sizeof(KT) == 8) a.K[j] = 0.0;
-            else a.K32[j] = 0.0f;
-        }
-    // ---- phase A: the block's uncut elements, once each ----
-    const int64_t e0 = a.blk_el_ptr[blk], ne = a.blk_el_ptr[blk + 1] - e0;
-    const double r0 = 0.57735026918962576450914878050196;
-    for (int64_t le = t; le < ne; le += ES_THREADS) {
-        const int e = a.blk_el[e0 + le];
-        const int4 cn = a.el_conn4[e];
-        const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
-        double X[4][2], u[8];
-        for (int q = 0; q < 4; ++q) {
-            const double2 xv = ((const double2*)a.Xq)[nd[q]];
-            X[q][0] = xv.x;
-            X[q][1] = xv.y;
-            if (DO_F)
-                for (int d = 0; d < 2; ++d) u[q * 2 + d] = a.U[d * a.n_all + nd[q]];
-        }
-        const Iso D = iso_of<2>(a.Dmat + a.el_mat[e] * 36);
-        double* geo = &s_geo[le * ES_GEO];
-#pragma unroll 1
-        for (int q = 0; q < 4; ++q) {
-            const double rr = (q % 2 == 0) ? -r0 : r0, ss = (q / 2 == 0) ? -r0 : r0;
-            const double dr[4][2] = {{-0.25 * (1.0 - ss), -0.25 * (1.0 - rr)},
-                                     {0.25 * (1.0 - ss), -0.25 * (1.0 + rr)},
-                                     {0.25 * (1.0 + ss), 0.25 * (1.0 + rr)},
-                                     {-0.25 * (1.0 + ss), 0.25 * (1.0 - rr)}};
-            double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
-            for (int k = 0; k < 4; ++k) {
-                J00 += dr[k][0] * X[k][0]; J01 += dr[k][0] * X[k][1];
-                J10 += dr[k][1] * X[k][0]; J11 += dr[k][1] * X[k][1];
-            }
-            const double det = J00 * J11 - J01 * J10;
-            const double rd = 1.0 / det;
-            const double i00 = J11 * rd, i01 = -J01 * rd, i10 = -J10 * rd, i11 = J00 * rd;
-            geo[q * 8 + 0] = i00;
-            geo[q * 8 + 1] = i01;
-            geo[q * 8 + 2] = i10;
-            geo[q * 8 + 3] = i11;
-            geo[q * 8 + 4] = det * 1.0;
-            if (DO_F) {
-                double g[8], eps[3], sig[3];
-                for (int k = 0; k < 4; ++k) {
-                    g[k * 2 + 0] = i00 * dr[k][0] + i01 * dr[k][1];
-                    g[k * 2 + 1] = i10 * dr[k][0] + i11 * dr[k][1];
-                }
-                strain_of<2>(4, g, u, eps);
-                stress_of<2>(D, eps, sig);
-                geo[q * 8 + 5] = sig[0];
-                geo[q * 8 + 6] = sig[1];
-                geo[q * 8 + 7] = sig[2];
-            }
-        }
-    }
-    __syncthreads();
-    // ---- phase B: each row over its incident elements, in element order ----
-    double Ptot = 0.0;
-    const int64_t kbeg = own ? a.inc_ptr[n] : 0, kend = own ? a.inc_ptr[n + 1] : 0;
-    for (int64_t k = kbeg; k < kend; ++k) {
-        const int2 rec = a.inc_rec[k];
-        const int la = rec.y & 0xff;
-        const int c = (rec.y & 0x100) ? rec.x : -1;
-        auto add = [&](int64_t rel, double v) {
-            if (in_lds) s_seg[base + rel - s0] += v;
-            else kadd<KT>(a, base + rel, v);
-        };
-        if (c < 0) {
-            const double* geo = &s_geo[(int64_t)a.inc_loc[k] * ES_GEO];
-            const Iso D = iso_of<2>(a.Dmat + (rec.y >> 16) * 36);
-            double Kr[8], Pc = 0.0;
-            for (int j = 0; j < 8; ++j) Kr[j] = 0.0;
-#pragma unroll 1
-            for (int q = 0; q < 4; ++q) {
-                const double rr = (q % 2 == 0) ? -r0 : r0, ss = (q / 2 == 0) ? -r0 : r0;
-                const double dr[4][2] = {{-0.25 * (1.0 - ss), -0.25 * (1.0 - rr)},
-                                         {0.25 * (1.0 - ss), -0.25 * (1.0 + rr)},
-                                         {0.25 * (1.0 + ss), 0.25 * (1.0 + rr)},
-                                         {-0.25 * (1.0 + ss), 0.25 * (1.0 - rr)}};
-                const double i00 = geo[q * 8 + 0], i01 = geo[q * 8 + 1], i10 = geo[q * 8 + 2], i11 = geo[q * 8 + 3];
-                const double w = geo[q * 8 + 4];
-                double g[8];
-                for (int j = 0; j < 4; ++j) {
-                    g[j * 2 + 0] = i00 * dr[j][0] + i01 * dr[j][1];
-                    g[j * 2 + 1] = i10 * dr[j][0] + i11 * dr[j][1];
-                }
-                double gr[2];
-                for (int i = 0; i < 2; ++i)
-                    gr[i] = la == 0 ? g[i] : la == 1 ? g[2 + i] : la == 2 ? g[4 + i] : g[6 + i];
-                if (DO_F) {
-                    const double sig[3] = {geo[q * 8 + 5], geo[q * 8 + 6], geo[q * 8 + 7]};
-                    Pc += w * force_entry<2>(gr, da, sig);
-                }
-                if (DO_K) {
-                    double parts[3];
-                    k_parts<2>(gr, da, D, parts);
-                    for (int bn = 0; bn < 4; ++bn)
-                        for (int db = 0; db < 2; ++db) Kr[bn * 2 + db] += w * k_entry<2>(parts, &g[bn * 2], db);
-                }
-            }
-            if (DO_F) Ptot += Pc;
-            if (DO_K) {
-                const uint32_t* rw = (const uint32_t*)(a.inc_rel + k * 8);
-                const uint32_t wv[2] = {rw[0], rw[1]};
-                for (int j = 0; j < 8; ++j) add((int)((wv[j / 4] >> (8 * (j % 4))) & 0xffu), Kr[j]);
-            }
-        } else {
-            if (DO_F) Ptot += a.pe[a.pe_ptr[c] + la * 2 + da];
-            if (DO_K) {
-                const int nt = a.ce_nt[c], M = nt * 2;
-                const uint8_t* rl = a.ce_rel + a.ce_rel_ptr[c] + (int64_t)la * M;
-                const double* kb = a.ke + a.ke_ptr[c] + (int64_t)(la * 2 + da) * M;
-                for (int j = 0; j < M; ++j) add(rl[j], kb[j]);
-            }
-        }
-    }
-    if (DO_F && own) a.P[r] = Ptot;
-    if (use_lds) {
-        __syncthreads();
-        for (int64_t j = t; j < s1 - s0; j += ES_THREADS) {
-            if (sizeof(KT) == 8) a.K[s0 + j] = s_seg[j];
-            else a.K32[s0 + j] = (float)s_seg[j];
-        }
-    }
-}
-
 // Gather scheme, pass 2.  A block owns a run of consecutive nodes.  Its
 // threads are (unit, slot) pairs: a unit is one matrix row (NR = 1) or all DIM
 // rows of one node (NR = DIM, the element geometry and the nodal data then
@@ -1240,12 +1084,6 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
         pgfk::AsmArgs ar = a;
         ar.gblk_lo = rg[0];
         ar.n_gblk = rg[1];
-        if (DIM == 2 && c.ges) {
-            const size_t lds_es = sizeof(double) * (size_t)(c.seg_max + c.es_geo_max * pgfk::ES_GEO);
-            hipLaunchKernelGGL((pgfk::k_gather_es<DO_K, DO_F, KT>), dim3((unsigned)pgfk::xcd_blocks(rg[1] - rg[0])),
-                               dim3(pgfk::ES_THREADS), lds_es, c.stream, ar);
-            continue;
-        }
         const dim3 grid((unsigned)pgfk::xcd_blocks(rg[1] - rg[0])), blk(pgfk::GATHER_THREADS);
 #define PGF_GATHER(SL, NR) hipLaunchKernelGGL((pgfk::k_gather<DIM, SL, NR, DO_K, DO_F, KT>), grid, blk, lds, c.stream, ar)
         if (c.gnode) {

@@ -90,7 +90,6 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index; a.image = image;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_rel = inc_rel;
     a.inc_rec = (const int2*)inc_rec; a.el_conn4 = (const int4*)el_conn4; a.Xq = Xq;
-    a.blk_el_ptr = blk_el_ptr; a.blk_el = blk_el; a.inc_loc = inc_loc; a.es_seg_cap = seg_max;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
     return a;
@@ -103,8 +102,7 @@ Ctx::~Ctx() {
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
                     inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row, ce_rel, ce_rel_ptr, image,
-                    sens_dgrad, sens_divv, sens_pe, sens_ds, sens_param, inc_rec, el_conn4, Xq, blk_el_ptr, blk_el,
-                    inc_loc};
+                    sens_dgrad, sens_divv, sens_pe, sens_ds, sens_param, inc_rec, el_conn4, Xq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -195,14 +193,10 @@ __device__ inline void sum_partials(int nb, const double* partial, double* v) {
 // divided by the slots (rounded up).
 int default_gather_slots(int dim) { return dim == 2 ? 1 : 2; }
 
-void build_gather_blocks(Ctx& c, int slots, int mode) {
+void build_gather_blocks(Ctx& c, int slots, bool node_rows) {
     if (slots != 1 && slots != 2 && slots != 4 && slots != 8)
         throw pgf::Error(PGF_EARG, "gather slots must be 1, 2, 4 or 8");
-    if (mode == 2 && (c.dim != 2 || slots != 1))
-        throw pgf::Error(PGF_EARG, "the element-staged gather is 2D with one slot per row");
-    const bool node_rows = mode == 1;
-    const int per_blk = mode == 2 ? pgfk::ES_THREADS / c.dim
-                                  : pgfk::GATHER_THREADS / (slots * (node_rows ? 1 : c.dim));
+    const int per_blk = pgfk::GATHER_THREADS / (slots * (node_rows ? 1 : c.dim));
     const int64_t na = c.n_all;
     const std::vector<int64_t>& rp = c.h_rowptr;
     std::vector<int64_t> bn{0}, bs;
@@ -252,41 +246,6 @@ void build_gather_blocks(Ctx& c, int slots, int mode) {
     c.seg_max = seg_max;
     c.gslots = slots;
     c.gnode = node_rows;
-    c.ges = mode == 2;
-    for (void* p : {(void*)c.blk_el_ptr, (void*)c.blk_el, (void*)c.inc_loc})
-        if (p) HIPCHK(hipFree(p));
-    c.blk_el_ptr = nullptr; c.blk_el = nullptr; c.inc_loc = nullptr;
-    c.es_geo_max = 0;
-    if (mode == 2) {
-        // each block's uncut elements in element order, and per incidence the
-        // element's position in its block's list
-        const int64_t nb = (int64_t)bn.size() - 1;
-        std::vector<int64_t> ep{0};
-        std::vector<int32_t> el;
-        std::vector<uint16_t> loc((size_t)std::max<int64_t>(c.n_inc, 1), 0);
-        std::vector<int32_t> tmp;
-        for (int64_t b = 0; b < nb; ++b) {
-            tmp.clear();
-            for (int64_t n = bn[b]; n < bn[b + 1]; ++n)
-                for (int64_t k = c.h_inc_ptr[n]; k < c.h_inc_ptr[n + 1]; ++k)
-                    if (c.h_cut_index[c.h_inc_elem[k]] < 0) tmp.push_back(c.h_inc_elem[k]);
-            std::sort(tmp.begin(), tmp.end());
-            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-            if (tmp.size() > 65535) throw pgf::Error(PGF_ELIMIT, "too many elements in one gather block");
-            for (int64_t n = bn[b]; n < bn[b + 1]; ++n)
-                for (int64_t k = c.h_inc_ptr[n]; k < c.h_inc_ptr[n + 1]; ++k)
-                    if (c.h_cut_index[c.h_inc_elem[k]] < 0)
-                        loc[k] = (uint16_t)(std::lower_bound(tmp.begin(), tmp.end(), c.h_inc_elem[k]) - tmp.begin());
-            el.insert(el.end(), tmp.begin(), tmp.end());
-            ep.push_back((int64_t)el.size());
-            c.es_geo_max = std::max<int64_t>(c.es_geo_max, (int64_t)tmp.size());
-        }
-        const size_t lds = sizeof(double) * (size_t)(seg_max + c.es_geo_max * pgfk::ES_GEO);
-        if (lds > 160 * 1024) throw pgf::Error(PGF_ELIMIT, "element-staged gather blocks exceed the LDS");
-        c.blk_el_ptr = dupload(ep.data(), (int64_t)ep.size());
-        c.blk_el = dupload(el.data(), std::max<int64_t>((int64_t)el.size(), 1));
-        c.inc_loc = dupload(loc.data(), (int64_t)loc.size());
-    }
 }
 
 // scalar slots of the PCG recurrence
@@ -867,10 +826,6 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->inc_ptr = pgfd::dupload(pat.inc_ptr.data(), (int64_t)pat.inc_ptr.size());
             c->inc_elem = pgfd::dupload(pat.inc_elem.data(), c->n_inc);
             c->inc_slot = pgfd::dupload(pat.inc_slot.data(), c->n_inc);
-            if (d->dim == 2) {
-                c->h_inc_elem.assign(pat.inc_elem.begin(), pat.inc_elem.end());
-                c->h_cut_index.assign(mv.cut_index.begin(), mv.cut_index.end());
-            }
             {
                 // packed records of the gather pass (one 8-byte load per
                 // incidence, one 16-byte load per element, one or two per node)
@@ -940,7 +895,7 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
                 c->image = pgfd::dupload(c->h_image.data(), d->n_all_nodes);
             }
             c->h_inc_ptr = pat.inc_ptr;
-            pgfd::build_gather_blocks(*c, pgfd::default_gather_slots(d->dim), 0);
+            pgfd::build_gather_blocks(*c, pgfd::default_gather_slots(d->dim), false);
         }
         c->dmax_conv = pgfd::dalloc<double>(n_cpt);
         c->dmax_trial = pgfd::dalloc<double>(n_cpt);
@@ -1424,10 +1379,10 @@ PGF_API int pgf_time_assemble_split(pgf_ctx* c, uint32_t flags, int32_t reps, do
 }
 
 PGF_API int pgf_set_gather_layout(pgf_ctx* c, int32_t slots, int32_t node_rows) {
-    if (!c || node_rows < 0 || node_rows > 2) return PGF_EARG;
+    if (!c || (node_rows != 0 && node_rows != 1)) return PGF_EARG;
     return guard(c, [&]() -> int {
         HIPCHK(hipStreamSynchronize(c->stream));
-        pgfd::build_gather_blocks(*c, slots, node_rows);
+        pgfd::build_gather_blocks(*c, slots, node_rows != 0);
         return 0;
     });
 }
@@ -1435,7 +1390,7 @@ PGF_API int pgf_set_gather_layout(pgf_ctx* c, int32_t slots, int32_t node_rows) 
 PGF_API int pgf_get_gather_layout(pgf_ctx* c, int32_t* slots, int32_t* node_rows) {
     if (!c || !slots || !node_rows) return PGF_EARG;
     *slots = c->gslots;
-    *node_rows = c->ges ? 2 : (c->gnode ? 1 : 0);
+    *node_rows = c->gnode ? 1 : 0;
     return 0;
 }
 

@@ -56,14 +56,6 @@ struct Ctx {
     int32_t* blk_rounds = nullptr;
     int64_t n_gblk = 0, seg_max = 0;
     int64_t n_gblk_reg = 0;         // gather blocks of the regular nodes (the rest: enrichment nodes)
-    // element-staged gather (2D): per block its uncut elements, per incidence
-    // the position in that list; host copies of the incidence to build them
-    bool ges = false;
-    int64_t es_geo_max = 0;         // largest element list of a block
-    int64_t* blk_el_ptr = nullptr;
-    int32_t* blk_el = nullptr;
-    uint16_t* inc_loc = nullptr;
-    std::vector<int32_t> h_inc_elem, h_cut_index;
     int gslots = 1;                 // gather pass: threads per row (or per node with gnode)
     bool gnode = false;             // gather pass: one thread serves all rows of a node
     int64_t n_vpt = 0, n_vgrad = 0;  // volumetric points of the cut elements and their gradient entries
@@ -114,7 +106,7 @@ double* noalloc_scratch();
 // nodal field U from a dof vector (free part x, prescribed part up or zero)
 void enqueue_set_from_dofs(Ctx& c, const double* d_x, const double* d_up);
 int default_gather_slots(int dim);
-void build_gather_blocks(Ctx& c, int slots, int mode);  // mode 0 rows, 1 nodes, 2 element-staged rows (2D)
+void build_gather_blocks(Ctx& c, int slots, bool node_rows);
 
 // Learned initial guess (pgf_cnn.hip).  Called after the Jacobian assembly of
 // Newton iteration `it` with that iteration's residual in c.r.  Iteration 0

@@ -29,8 +29,6 @@ __host__ __device__ constexpr int cut_threads() { return 64; }
 // row-gather blocks: threads (slots x matrix rows) and LDS segment (doubles)
 constexpr int GATHER_THREADS = 256;
 constexpr int SEG_MAX = 4096;
-constexpr int ES_THREADS = 128;  // element-staged gather block (2D): one row per thread
-constexpr int ES_GEO = 32;       // doubles per staged QUAD4: 4 points x (inverse J (4), J*W, stress (3))
 // MI355X dispatches consecutive workgroups to its 8 XCDs in turn, each XCD
 // with its own L2.  Launching 8*per blocks and mapping block b to logical
 // block (b % 8) * per + b / 8 gives every XCD one contiguous run of logical
@@ -100,12 +98,6 @@ struct AsmArgs {
     const int2* inc_rec;       // [n_inc]
     const int4* el_conn4;      // [n_elem]
     const double* Xq;          // [n_all][2] (2D) or [n_all][4] (3D)
-    // element-staged gather (2D): per block its uncut elements, per incidence
-    // the element's position in its block's list, LDS offset of the tables
-    const int64_t* blk_el_ptr; // [n_gblk+1]
-    const int32_t* blk_el;
-    const uint16_t* inc_loc;   // [n_inc]
-    int64_t es_seg_cap;        // doubles of LDS before the element tables
     double* ke;                // cut-element K_e scratch
     const int64_t* ke_ptr;
     double* pe;                // cut-element P_e scratch

@@ -141,9 +141,9 @@ def test_3d_cohesive_stiffness(law, scheme):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_gather_layouts_bit_identical(dim):
     """Every layout of the gather pass (1, 2, 4, 8 threads per unit; a unit one
-    matrix row or all rows of a node; in 2D also the element-staged rows) sums
-    each entry's element contributions in element order: K, P and the cohesive
-    state are bit-identical across layouts, and match the oracle."""
+    matrix row or all rows of a node) sums each entry's element contributions
+    in element order: K, P and the cohesive state are bit-identical across
+    layouts, and match the oracle."""
     p, o = _c0(60) if dim == 2 else _c2small(configs.BILINEAR_C2)
     ctx = DeviceContext(p)
     rng = np.random.RandomState(5)
@@ -154,8 +154,6 @@ def test_gather_layouts_bit_identical(dim):
     rp = ctx.pattern()[0]
     ref = None
     layouts = [(slots, node_rows) for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
-    if dim == 2:
-        layouts.append((1, 2))  # element-staged rows
     for slots, node_rows in layouts:
         if True:
             ctx.gather_layout = (slots, node_rows)
@@ -175,10 +173,7 @@ def test_gather_layouts_bit_identical(dim):
     with pytest.raises(RuntimeError):
         ctx.gather_layout = (3, 0)
     with pytest.raises(RuntimeError):
-        ctx.gather_layout = (1, 3)
-    if dim == 3:
-        with pytest.raises(RuntimeError):
-            ctx.gather_layout = (1, 2)  # element staging is 2D only
+        ctx.gather_layout = (1, 2)
 
 
 @pytest.mark.parametrize("precision", [64, 32])
