@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--c3-nmin", type=int, default=64)
     ap.add_argument("--c3-nmax", type=int, default=512)
     ap.add_argument("--c3-slots", type=int, default=1, help="inclusion radii per RVE with direct sensitivities")
+    ap.add_argument("--setup", choices=["device", "host"], default="device",
+                    help="where the IGFEM preprocessing (detection, enrichment, integration tables) runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -316,15 +318,16 @@ def learned_vs_plain(n, n_incl, steps, prob=None, train_n=64, train_incl=20, tra
     from pigfem_assignment_amd.api import GuessNet
     t0 = time.time()
     net = GuessNet(seed=3)
-    train = [learn.cohesive_problem(train_n, train_incl, seed) for seed in (501, 502)]
-    loss = learn.train_net(net, train, train_steps, 0.05, epochs=epochs, lr=1e-3, log=log)
-    t_train = time.time() - t0
+    out = {"learn_workload": "unit cube %d^3 x 6 TET4, %d spheres, bilinear cohesive, first %d of 20 increments" % (
+        n, n_incl, steps)}
+    if any(ALL_MODES[m][0] == 2 for m in modes):  # the network is trained only when a mode uses it
+        train = [learn.cohesive_problem(train_n, train_incl, seed) for seed in (501, 502)]
+        loss = learn.train_net(net, train, train_steps, 0.05, epochs=epochs, lr=1e-3, log=log)
+        out.update({"learn_train_s": time.time() - t0, "learn_loss_first": float(loss[0]),
+                    "learn_loss_last": float(loss[-1])})
     if prob is None:
         prob = (configs.c2_problem(n=n, n_incl=n_incl), configs.c2_inclusions(n_incl, n))
     dfrac = 0.05  # 20 increments to 2 % strain
-    out = {"learn_train_s": t_train, "learn_loss_first": float(loss[0]), "learn_loss_last": float(loss[-1]),
-           "learn_workload": "unit cube %d^3 x 6 TET4, %d spheres, bilinear cohesive, first %d of 20 increments" % (
-               n, n_incl, steps)}
     modes = [(m,) + ALL_MODES[m] for m in modes]
     flags, sols = {}, {}
     for name, ug, prec, lin, adapt, m32 in modes:
@@ -402,12 +405,13 @@ def bench_c2(args):
     from pigfem_assignment_amd import configs, _lib as L
     from pigfem_assignment_amd.api import DeviceContext
     t0 = time.time()
-    prob = configs.c2_problem(n=args.c2_n, n_incl=args.c2_incl)
+    prob = configs.c2_problem(n=args.c2_n, n_incl=args.c2_incl, device_preprocess=args.setup == "device")
     incl = configs.c2_inclusions(args.c2_incl, args.c2_n)
     info = prob.info
     setup_s = time.time() - t0
-    print("# C2 setup: %d tets (%d cut), %d dofs, %d cohesive points, host %.1f s" % (
-        info.n_elem, info.n_cut, info.n_free, info.n_cpt, setup_s), flush=True)
+    print("# C2 setup: %d tets (%d cut), %d dofs, %d cohesive points, set-up %.1f s (IGFEM phases on the %s: "
+          "nodes %.1f, elements %.1f, enrichment %.1f, tables %.1f ms)" % (
+              info.n_elem, info.n_cut, info.n_free, info.n_cpt, setup_s, args.setup, *info.phase_ms), flush=True)
     ctx = DeviceContext(prob)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
     slots, slot_ms = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
@@ -435,7 +439,8 @@ def bench_c2(args):
                  "algorithmic_bytes": bytes_cut, "traffic": measured_traffic("c2", "k_cut_elem_mfma")}
     extra = {"assembly_ms": ms, "cut_pass_ms": ms_cut, "gather_pass_ms": ms_gather, "cut_pass_bytes": bytes_cut,
              "gather_pass_bytes": bytes_gather, "gather_layout": slots, "gather_pass_ms_by_layout": slot_ms,
-             "roofline_other_pass": other, "host_setup_s": setup_s, "cohesive_points": info.n_cpt}
+             "roofline_other_pass": other, "setup_s": setup_s, "setup_igfem_on": args.setup,
+             "setup_igfem_phase_ms": list(info.phase_ms), "cohesive_points": info.n_cpt}
     # K_ff SpMV of the Newton solves: FP64 values, and FP32 values with FP64 vectors
     ctx.assemble(L.PGF_ASM_JAC, dt=0.05)
     ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FP32, dt=0.05)
@@ -502,11 +507,17 @@ def bench_c3(args):
     rng = np.random.RandomState(20261202)
     sizes = rng.randint(args.c3_nmin, args.c3_nmax + 1, size=args.c3_rves)
     t0 = time.time()
-    parts, incls = zip(*[batch.periodic_rve_problem(int(N), 1000 + k) for k, N in enumerate(sizes)])
-    B = batch.BatchProblem(list(parts))
+    parts = []
+    for k, N in enumerate(sizes):
+        parts.append(batch.periodic_rve_problem(int(N), 1000 + k, device_preprocess=args.setup == "device")[0])
+        if k % 64 == 63:
+            print("# C3 set-up: %d of %d RVEs built, %.1f s" % (k + 1, len(sizes), time.time() - t0), flush=True)
+    B = batch.BatchProblem(parts)
     setup_s = time.time() - t0
-    print("# C3 set-up: %d periodic RVEs, %d elements (%d cut), %d dofs, host %.1f s" % (
-        len(parts), B.n_elem, B.n_cut, B.n_free, setup_s), flush=True)
+    phases = np.sum([p.info.phase_ms[:] for p in parts], axis=0)
+    print("# C3 set-up: %d periodic RVEs, %d elements (%d cut), %d dofs, set-up %.1f s (IGFEM phases on the %s, "
+          "summed: nodes %.0f, elements %.0f, enrichment %.0f, tables %.0f ms)" % (
+              len(parts), B.n_elem, B.n_cut, B.n_free, setup_s, args.setup, *phases), flush=True)
     ctx = DeviceContext(B)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS
     slots, _ = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
@@ -522,7 +533,8 @@ def bench_c3(args):
                                                  its.min(), its.max(), tune), flush=True)
     extra = {"batched_solve_ms": ms_solve, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
              "iterations_total": int(its.sum()), "autotuned": list(tune), "gather_layout": slots,
-             "host_setup_s": setup_s, "C_eff_mean": Ce.mean(axis=0).tolist()}
+             "setup_s": setup_s, "setup_igfem_on": args.setup, "setup_igfem_phase_ms": phases.tolist(),
+             "C_eff_mean": Ce.mean(axis=0).tolist()}
     if args.c3_slots > 0:
         t1 = time.time()
         sens.attach(ctx, *sens.batch_tables(list(parts)))

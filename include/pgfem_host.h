@@ -61,6 +61,10 @@ typedef struct pgfh_config {
      * conditions count (DofObject::set_local_owned_normal_dofs,
      * src/DofObject.cpp:336-386) */
     int32_t periodic;
+    /* 1: node detection, element detection, enrichment nodes and the cut
+     * elements' integration tables run on the current HIP device (same
+     * results as the host, bit for bit); 0: on the host */
+    int32_t device_preprocess;
 } pgfh_config;
 
 typedef struct pgfh_info {
@@ -69,6 +73,10 @@ typedef struct pgfh_info {
     int64_t n_nodes, n_all_nodes, n_elem, n_cut, n_free, n_pres;
     int64_t n_vpt, n_vgrad, n_cpt, n_moved_nodes, n_enrich_initial;
     double setup_ms;
+    /* wall time of the IGFEM phases: node detection, element detection,
+     * enrichment nodes, integration tables (device transfers included) */
+    double phase_ms[4];
+    int32_t device_preprocess;  /* 1 when the phases ran on the device */
 } pgfh_info;
 
 PGF_API int pgfh_problem_create(const pgfh_config* cfg, pgfh_problem** out);

@@ -68,7 +68,7 @@ class HostConfig(C.Structure):
         ("n_coh_law", C.c_int32), ("coh_type", i32p), ("coh_params", dp), ("default_coh_law", C.c_int32),
         ("sphere_zlim_as_reference", C.c_int32), ("tet_weight_scale", C.c_double),
         ("n_bc", C.c_int32), ("bc_set", i32p), ("bc_node", i32p), ("bc_dof", i32p), ("bc_val", dp),
-        ("periodic", C.c_int32),
+        ("periodic", C.c_int32), ("device_preprocess", C.c_int32),
     ]
 
 
@@ -76,7 +76,8 @@ class HostInfo(C.Structure):
     _fields_ = [("dim", C.c_int32), ("n_colors_hint", C.c_int32), ("n_nodes", C.c_int64),
                 ("n_all_nodes", C.c_int64), ("n_elem", C.c_int64), ("n_cut", C.c_int64), ("n_free", C.c_int64),
                 ("n_pres", C.c_int64), ("n_vpt", C.c_int64), ("n_vgrad", C.c_int64), ("n_cpt", C.c_int64),
-                ("n_moved_nodes", C.c_int64), ("n_enrich_initial", C.c_int64), ("setup_ms", C.c_double)]
+                ("n_moved_nodes", C.c_int64), ("n_enrich_initial", C.c_int64), ("setup_ms", C.c_double),
+                ("phase_ms", C.c_double * 4), ("device_preprocess", C.c_int32)]
 
 
 class LoadOpts(C.Structure):

@@ -15,7 +15,8 @@ class HostProblem:
 
     def __init__(self, kind, N, mins, maxs, inclusions=(), materials=((3.2e9, 0.35),), incl_mat=None,
                  plane_strain=True, cohesive=False, coh_laws=(), incl_coh_law=None, default_coh_law=0,
-                 bcs=(), tet_weight_scale=1.0, sphere_zlim_as_reference=False, periodic=False):
+                 bcs=(), tet_weight_scale=1.0, sphere_zlim_as_reference=False, periodic=False,
+                 device_preprocess=False):
         lib = L.lib()
         self._keep = []
         cfg = L.HostConfig()
@@ -58,6 +59,7 @@ class HostProblem:
         cfg.bc_set, cfg.bc_node = L.ptr(bset, C.c_int32), L.ptr(bnode, C.c_int32)
         cfg.bc_dof, cfg.bc_val = L.ptr(bdof, C.c_int32), L.ptr(bval, C.c_double)
         cfg.periodic = int(bool(periodic))
+        cfg.device_preprocess = int(bool(device_preprocess))
         self._keep += [inc, im, icl, E, nu, ct, cp, bset, bnode, bdof, bval]
         h = C.c_void_p()
         L.check(lib.pgfh_problem_create(C.byref(cfg), C.byref(h)), host=True)
@@ -152,6 +154,29 @@ class HostProblem:
         d = L.MeshDesc()
         L.check(L.lib().pgfh_mesh_desc(self.h, C.byref(d)), host=True)
         return d
+
+    def tables(self):
+        """Copies of the cut elements' integration tables (store_shape_functions)
+        as a dict of arrays."""
+        d = self.desc()
+        nc = d.n_cut
+
+        def arr(ptr, n, ct):
+            if n <= 0:
+                return np.zeros(0, np.dtype(ct))
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(n,)).copy()
+
+        vptr = arr(d.vpt_ptr, nc + 1, C.c_int64)
+        cptr = arr(d.cpt_ptr, nc + 1, C.c_int64)
+        nv, ncp = int(vptr[-1]), int(cptr[-1])
+        return {"cut_elem": arr(d.cut_elem, nc, C.c_int32), "cut_n_enrich": arr(d.cut_n_enrich, nc, C.c_int32),
+                "cut_enrich": arr(d.cut_enrich, 16 * nc, C.c_int32), "vpt_ptr": vptr,
+                "vpt_w": arr(d.vpt_w, nv, C.c_double), "vpt_mat": arr(d.vpt_mat, nv, C.c_int32),
+                "vpt_grad_off": arr(d.vpt_grad_off, nv, C.c_int64),
+                "vpt_grad": arr(d.vpt_grad, int(self.info.n_vgrad), C.c_double), "cpt_ptr": cptr,
+                "cpt_w": arr(d.cpt_w, ncp, C.c_double), "cpt_N": arr(d.cpt_N, 4 * ncp, C.c_double),
+                "cpt_R": arr(d.cpt_R, 9 * ncp, C.c_double), "cpt_nodes": arr(d.cpt_nodes, 8 * ncp, C.c_uint8),
+                "cpt_nh": arr(d.cpt_nh, ncp, C.c_uint8), "cpt_law": arr(d.cpt_law, ncp, C.c_int32)}
 
     def write_vtk(self, path, U=None, eps_bar=None, t=0.0):
         """legacy ASCII VTK file: mesh (cut elements as their sub-elements),

@@ -70,12 +70,13 @@ def rve_problem(N, seed, materials=(configs.MATRIX, configs.INCLUSION), inclusio
     return HostProblem("quad4", (N, N), (0, 0), (1, 1), inclusions=incl, materials=materials, bcs=bcs), incl
 
 
-def periodic_rve_problem(N, seed, materials=(configs.MATRIX, configs.INCLUSION), inclusions=None):
+def periodic_rve_problem(N, seed, materials=(configs.MATRIX, configs.INCLUSION), inclusions=None,
+                         device_preprocess=False):
     """Periodic N x N QUAD4 RVE with node 0 (all corners) fixed."""
     incl = rve_inclusions(N, seed) if inclusions is None else inclusions
     bcs = [(L.SET_NODE, 0, 0, 0.0), (L.SET_NODE, 0, 1, 0.0)]
     return HostProblem("quad4", (N, N), (0, 0), (1, 1), inclusions=incl, materials=materials, bcs=bcs,
-                       periodic=True), incl
+                       periodic=True, device_preprocess=device_preprocess), incl
 
 
 class BatchProblem:

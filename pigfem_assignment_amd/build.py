@@ -19,7 +19,10 @@ DRIVER = os.path.join(HERE, "pgf_driver")
 HOST_SRC = ["host/pgf_mesh.cpp", "host/pgf_dofs.cpp", "host/pgf_problem.cpp", "host/pgf_sens.cpp",
             "host/pgf_writer.cpp", "host/pgf_batch_host.cpp"]
 HIP_SRC = ["kernels/pgf_assemble.hip", "kernels/pgf_device.hip", "kernels/pgf_cnn.hip", "kernels/pgf_batch.hip",
-           "kernels/pgf_sens.hip"]
+           "kernels/pgf_sens.hip", "kernels/pgf_preprocess.hip"]
+# the device set-up shares its arithmetic with the host set-up (g++, no FMA on
+# x86-64): no multiply-add contraction, so both round every operation alike
+HIP_EXTRA = {"kernels/pgf_preprocess.hip": ["-ffp-contract=off"]}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -46,7 +49,7 @@ def build(verbose=False):
     for s in HIP_SRC:
         o = os.path.join(OBJ, os.path.basename(s) + ".o")
         objs.append(o)
-        jobs.append([HIPCC] + HIPFLAGS + ["-x", "hip", "-c", os.path.join(CSRC, s), "-o", o])
+        jobs.append([HIPCC] + HIPFLAGS + HIP_EXTRA.get(s, []) + ["-x", "hip", "-c", os.path.join(CSRC, s), "-o", o])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:

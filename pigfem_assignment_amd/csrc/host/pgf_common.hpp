@@ -11,6 +11,15 @@
 #include <vector>
 #include <stdexcept>
 
+// Functions marked PGF_HD are compiled for the host (g++) and, from the HIP
+// set-up kernels, for the device; that code is built without contraction of
+// multiply-adds so both sides round every operation the same way.
+#if defined(__HIPCC__)
+#define PGF_HD __host__ __device__
+#else
+#define PGF_HD
+#endif
+
 namespace pgf {
 
 enum Status : int {

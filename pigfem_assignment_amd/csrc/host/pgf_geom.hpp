@@ -30,36 +30,36 @@ constexpr double kPI = 3.14159265358979323846;
 constexpr double kGauss2 = 0.57735026918962576450914878050196;
 
 // ---- reference rules ------------------------------------------------------
-inline void edge2_qp(int qp, double& xi, double& w) {
+PGF_HD inline void edge2_qp(int qp, double& xi, double& w) {
     xi = qp == 0 ? -kGauss2 : kGauss2;
     w = 1.0;
 }
-inline void quad4_qp(int qp, double* xi, double& w) {
+PGF_HD inline void quad4_qp(int qp, double* xi, double& w) {
     double w0, w1;
     edge2_qp(qp % 2, xi[0], w0);
     edge2_qp(qp / 2, xi[1], w1);
     w = w0 * w1;
 }
-inline void tri3_qp(int qp, double* xi, double& w) {
-    static const double P[4][2] = {{0.333333333333333333333333333333333, 0.333333333333333333333333333333333},
-                                   {0.2, 0.2}, {0.2, 0.6}, {0.6, 0.2}};
-    static const double W[4] = {-0.28125, 0.260416666666666666666666666666665,
-                                0.260416666666666666666666666666665, 0.260416666666666666666666666666665};
+PGF_HD inline void tri3_qp(int qp, double* xi, double& w) {
+    constexpr double P[4][2] = {{0.333333333333333333333333333333333, 0.333333333333333333333333333333333},
+                                {0.2, 0.2}, {0.2, 0.6}, {0.6, 0.2}};
+    constexpr double W[4] = {-0.28125, 0.260416666666666666666666666666665,
+                             0.260416666666666666666666666666665, 0.260416666666666666666666666666665};
     xi[0] = P[qp][0]; xi[1] = P[qp][1]; w = W[qp];
 }
-inline void tet4_qp(int qp, double* xi, double& w) {
-    static const double P[5][3] = {{0.25, 0.25, 0.25},
-                                   {0.5, 0.166666666666667, 0.166666666666667},
-                                   {0.166666666666667, 0.166666666666667, 0.166666666666667},
-                                   {0.166666666666667, 0.166666666666667, 0.5},
-                                   {0.166666666666667, 0.5, 0.166666666666667}};
-    static const double W[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
+PGF_HD inline void tet4_qp(int qp, double* xi, double& w) {
+    constexpr double P[5][3] = {{0.25, 0.25, 0.25},
+                                {0.5, 0.166666666666667, 0.166666666666667},
+                                {0.166666666666667, 0.166666666666667, 0.166666666666667},
+                                {0.166666666666667, 0.166666666666667, 0.5},
+                                {0.166666666666667, 0.5, 0.166666666666667}};
+    constexpr double W[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
     xi[0] = P[qp][0]; xi[1] = P[qp][1]; xi[2] = P[qp][2]; w = W[qp];
 }
 constexpr int kQuad4NQP = 4, kTri3NQP = 4, kTet4NQP = 5, kEdge2NQP = 2;
 
 // ---- shape functions (parent coordinates) -----------------------------------
-inline void quad4_shape(const double* xi, double* N, double (*dN)[3]) {
+PGF_HD inline void quad4_shape(const double* xi, double* N, double (*dN)[3]) {
     double r = xi[0], s = xi[1];
     N[0] = 0.25 * (1.0 - r) * (1.0 - s); N[1] = 0.25 * (1.0 + r) * (1.0 - s);
     N[2] = 0.25 * (1.0 + r) * (1.0 + s); N[3] = 0.25 * (1.0 - r) * (1.0 + s);
@@ -70,7 +70,7 @@ inline void quad4_shape(const double* xi, double* N, double (*dN)[3]) {
         dN[3][0] = -0.25 * (1.0 + s); dN[3][1] = 0.25 * (1.0 - r);
     }
 }
-inline void tri3_shape(const double* xi, double* N, double (*dN)[3]) {
+PGF_HD inline void tri3_shape(const double* xi, double* N, double (*dN)[3]) {
     N[0] = 1.0 - xi[0] - xi[1]; N[1] = xi[0]; N[2] = xi[1];
     if (dN) {
         dN[0][0] = -1.0; dN[0][1] = -1.0;
@@ -78,7 +78,7 @@ inline void tri3_shape(const double* xi, double* N, double (*dN)[3]) {
         dN[2][0] = 0.0;  dN[2][1] = 1.0;
     }
 }
-inline void tet4_shape(const double* xi, double* N, double (*dN)[3]) {
+PGF_HD inline void tet4_shape(const double* xi, double* N, double (*dN)[3]) {
     N[0] = 1.0 - xi[0] - xi[1] - xi[2]; N[1] = xi[0]; N[2] = xi[1]; N[3] = xi[2];
     if (dN) {
         dN[0][0] = -1.0; dN[0][1] = -1.0; dN[0][2] = -1.0;
@@ -90,17 +90,33 @@ inline void tet4_shape(const double* xi, double* N, double (*dN)[3]) {
 
 // Wachspress coordinates on the regular reference n-gon whose vertex k sits at
 // angle 2*pi*(k+1)/n on the unit circle; alpha_k = 1/(A_{k-1} A_k) with A_k
-// the signed area of triangle (x, p_k, p_{k+1}).
+// the signed area of triangle (x, p_k, p_{k+1}).  The vertices come from a
+// table V[k] = polygon_vertex(nn, k) built on the host, so host and device
+// use the same bits.
 inline void polygon_vertex(int nn, int k, double* p) {
     p[0] = std::cos(2.0 * kPI * (k + 1) / nn);
     p[1] = std::sin(2.0 * kPI * (k + 1) / nn);
 }
-inline void polygon_shape(int nn, const double* x, double* N, double (*dN)[3]) {
+// Vertex tables of the n-gons the sub-elements use (n = 0..MAX_EL_NODES;
+// row n holds n vertices).
+struct PolygonTable {
+    double v[MAX_EL_NODES + 1][MAX_EL_NODES][2];
+};
+inline const PolygonTable& polygon_table() {
+    static const PolygonTable t = [] {
+        PolygonTable p{};
+        for (int n = 3; n <= MAX_EL_NODES; ++n)
+            for (int k = 0; k < n; ++k) polygon_vertex(n, k, p.v[n][k]);
+        return p;
+    }();
+    return t;
+}
+PGF_HD inline void polygon_shape(int nn, const double (*V)[2], const double* x, double* N, double (*dN)[3]) {
     double A[MAX_EL_NODES + 1], dA[MAX_EL_NODES + 1][2];
     for (int n = 0; n < nn; ++n) {
         double p0[2] = {x[0], x[1]}, p1[2], p2[2];
-        polygon_vertex(nn, n, p1);
-        polygon_vertex(nn, (n + 1) % nn, p2);
+        p1[0] = V[n][0]; p1[1] = V[n][1];
+        p2[0] = V[(n + 1) % nn][0]; p2[1] = V[(n + 1) % nn][1];
         A[n + 1] = 0.5 * ((p1[0] * p2[1] - p1[1] * p2[0]) - (p0[0] * p2[1] - p0[1] * p2[0]) +
                           (p0[0] * p1[1] - p0[1] * p1[0]));
         dA[n + 1][0] = 0.5 * (p2[1] - p1[1]);
@@ -124,17 +140,18 @@ inline void polygon_shape(int nn, const double* x, double* N, double (*dN)[3]) {
         }
     }
 }
+inline void polygon_shape(int nn, const double* x, double* N, double (*dN)[3]) {
+    polygon_shape(nn, polygon_table().v[nn], x, N, dN);
+}
 // Fan quadrature point qp of the regular n-gon: triangle qp/4 = (origin, p_k,
 // p_{k+1}) with the 4-point TRI3 rule (src/Polygon.cpp:82-126).
-inline void polygon_qp(int nn, int qp, double* xi, double& w) {
+PGF_HD inline void polygon_qp(int nn, const double (*V)[2], int qp, double* xi, double& w) {
     double txi[2], tw;
     tri3_qp(qp % kTri3NQP, txi, tw);
     double tN[3], tdN[3][3];
     tri3_shape(txi, tN, tdN);
     int t = qp / kTri3NQP;
-    double pT[3][2] = {{0.0, 0.0}, {0, 0}, {0, 0}};
-    polygon_vertex(nn, t, pT[1]);
-    polygon_vertex(nn, (t + 1) % nn, pT[2]);
+    double pT[3][2] = {{0.0, 0.0}, {V[t][0], V[t][1]}, {V[(t + 1) % nn][0], V[(t + 1) % nn][1]}};
     double J0[2][2] = {{0, 0}, {0, 0}};
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
@@ -143,10 +160,11 @@ inline void polygon_qp(int nn, int qp, double* xi, double& w) {
     xi[0] = 0.0; xi[1] = 0.0;
     for (int n = 0; n < 3; ++n) { xi[0] += tN[n] * pT[n][0]; xi[1] += tN[n] * pT[n][1]; }
 }
+inline void polygon_qp(int nn, int qp, double* xi, double& w) { polygon_qp(nn, polygon_table().v[nn], qp, xi, w); }
 
 // Elem::transform_gradients (src/elem.cpp:123-136): Jac(a,i) = sum_n dNdr[n][a] x_n[i],
 // J = det(Jac), dNdx[n] = Jac^{-1} dNdr[n].  Returns J.
-inline double transform_gradients(int dim, int nn, const double (*X)[3], const double (*dNdr)[3],
+PGF_HD inline double transform_gradients(int dim, int nn, const double (*X)[3], const double (*dNdr)[3],
                                   double (*dNdx)[3]) {
     double Jc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int a = 0; a < dim; ++a)

@@ -63,6 +63,9 @@ struct CutElem {
     int32_t enrich[16] = {0};    // enrichment-node index per enrichment slot
 };
 
+// the device set-up writes these records with the same layout
+static_assert(sizeof(CutElem) == 224, "CutElem layout");
+
 struct MeshOptions {
     bool cohesive = false;
     bool plane_strain = true;
@@ -77,6 +80,14 @@ struct MeshOptions {
     double tet_weight_scale = 1.0;
     int default_coh_law = 0;
 };
+
+class Mesh;
+// IGFEM preprocessing on the current HIP device (kernels/pgf_preprocess.hip):
+// fills the members analyze_nodes, analyze_elements, add_enrichments and
+// store_shape_functions fill, with the same bits; phase_ms[4] receives the
+// wall time of the node, element, enrichment and table phases (transfers
+// included).
+void device_preprocess(Mesh& m, double* phase_ms);
 
 class Mesh {
    public:
@@ -152,8 +163,9 @@ class Mesh {
    private:
     int detect_node(int64_t n, int& times_moved);
     void move_node(int64_t n, const Inclusion& inc);
-    void detect_quad4(int64_t e, const int* nd, CutElem& ce, bool& cut);
-    void detect_tet4(int64_t e, const int* nd, CutElem& ce, bool& cut);
+    void detect_element(int64_t e, const int* nd, CutElem& ce, bool& cut);
+    // the device set-up (kernels/pgf_preprocess.hip) fills the same members
+    friend struct DevicePreprocess;
     void edge_nodes(int edge, int& a, int& b) const;
 
     int _dim = 0, _nn = 0;

@@ -39,6 +39,8 @@ struct pgfh_problem {
     std::vector<double> coords;  // [3][n_all]
     int plane_strain = 1;
     double setup_ms = 0.0;
+    double phase_ms[4] = {0, 0, 0, 0};
+    int device_preprocess = 0;
     int64_t n_enrich_initial = 0;
     pgf::Pattern pattern;
     bool have_pattern = false;
@@ -84,10 +86,22 @@ PGF_API int pgfh_problem_create(const pgfh_config* cfg, pgfh_problem** out) {
             if (inc.r <= 0) throw pgf::Error(PGF_EARG, "inclusion radius must be positive");
             p->mesh.add_inclusion(inc);
         }
-        p->mesh.analyze_nodes();
-        p->mesh.analyze_elements();
-        p->mesh.add_enrichments();
-        p->mesh.store_shape_functions();
+        if (cfg->device_preprocess) {
+            pgf::device_preprocess(p->mesh, p->phase_ms);
+            p->device_preprocess = 1;
+        } else {
+            auto ms = [] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+            double a = ms();
+            p->mesh.analyze_nodes();
+            double b = ms();
+            p->mesh.analyze_elements();
+            double c = ms();
+            p->mesh.add_enrichments();
+            double d = ms();
+            p->mesh.store_shape_functions();
+            double e = ms();
+            p->phase_ms[0] = b - a; p->phase_ms[1] = c - b; p->phase_ms[2] = d - c; p->phase_ms[3] = e - d;
+        }
         p->n_enrich_initial = opt.cohesive ? p->mesh.n_enrich_nodes() / 2 : p->mesh.n_enrich_nodes();
         // Dirichlet conditions (BoundaryObject::set_dirichlet_bcs_from_nodeset)
         std::vector<pgf::Dirichlet> bcs;
@@ -146,6 +160,8 @@ PGF_API int pgfh_problem_info(pgfh_problem* p, pgfh_info* info) {
     info->n_moved_nodes = m.n_moved_nodes();
     info->n_enrich_initial = p->n_enrich_initial;
     info->setup_ms = p->setup_ms;
+    for (int k = 0; k < 4; ++k) info->phase_ms[k] = p->phase_ms[k];
+    info->device_preprocess = p->device_preprocess;
     return 0;
 }
 
