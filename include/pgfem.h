@@ -63,7 +63,9 @@ extern "C" {
 /* pgf_assemble flags (assembleFunc / assembleJac of the reference) */
 #define PGF_ASM_FUNC 1u      /* internal force P_int, FP64 */
 #define PGF_ASM_JAC 2u       /* stiffness K */
-#define PGF_ASM_FP32 4u      /* K in FP32 (mixed-precision path); P_int stays FP64 */
+#define PGF_ASM_FP32 4u      /* K in FP32 (mixed-precision path), each entry the FP64 sum rounded once;
+                                the FP64 matrix is scratch for it (read it after an FP64 assembly);
+                                P_int stays FP64 */
 #define PGF_ASM_ALL_ROWS 8u  /* also assemble prescribed rows (K_pf, K_pp); default free rows only */
 #define PGF_ASM_COLORED 16u  /* element-coloured read-modify-write scheme instead of the row gather */
 

@@ -813,13 +813,13 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     }
     if (use_lds)
         for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) s_seg[j] = 0.0;
+    // rows outside the LDS segment are summed in FP64 in HBM (the FP64 array
+    // is that sum's scratch in FP32 mode), so an FP32 entry is always the
+    // FP64 sum rounded once, whatever the layout
     if (DO_K && slot == 0)
         for (int h = 0; h < NR; ++h)
             if (own[h] && !in_lds[h])
-                for (int32_t j = a.rowptr[r[h]]; j < a.rowptr[r[h] + 1]; ++j) {
-                    if (sizeof(KT) == 8) a.K[j] = 0.0;
-                    else a.K32[j] = 0.0f;
-                }
+                for (int32_t j = a.rowptr[r[h]]; j < a.rowptr[r[h] + 1]; ++j) a.K[j] = 0.0;
     if (DO_F && slot == 0)
         for (int h = 0; h < NR; ++h) s_P[lu * NR + h] = 0.0;
     __syncthreads();
@@ -900,7 +900,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                 if (DO_K) {
                     auto add = [&](int h, int64_t rel, double v) {
                         if (in_lds[h]) s_seg[base[h] + rel - s0] += v;
-                        else kadd<KT>(a, base[h] + rel, v);
+                        else a.K[base[h] + rel] += v;
                     };
                     if (c < 0) {
                         // row-relative column positions, one byte per column
@@ -930,6 +930,11 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     if (DO_F && slot == 0)
         for (int h = 0; h < NR; ++h)
             if (own[h]) a.P[r[h]] = s_P[lu * NR + h];
+    // (with several slots the last round ended on a barrier: every add is done)
+    if (DO_K && sizeof(KT) == 4 && slot == 0)
+        for (int h = 0; h < NR; ++h)
+            if (own[h] && !in_lds[h])
+                for (int32_t j = a.rowptr[r[h]]; j < a.rowptr[r[h] + 1]; ++j) a.K32[j] = (float)a.K[j];
     if (use_lds) {
         __syncthreads();
         for (int64_t j = t; j < s1 - s0; j += GATHER_THREADS) {
@@ -937,6 +942,13 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
             else a.K32[s0 + j] = (float)s_seg[j];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 sums rounded to the FP32 matrix (coloured scheme in FP32 mode)
+__global__ void k_round32(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1109,14 +1121,14 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
 template <int DIM>
 static void launch_dim(Ctx& c, const pgfk::AsmArgs& a, bool jac, bool func, bool fp32, bool colored) {
     if (colored) {
-        if (jac) {
-            if (fp32) fill_async(c, c.K32, c.nnz, 0.0f);
-            else fill_async(c, c.K, c.nnz, 0.0);
-        }
+        // FP32 mode sums in the FP64 array and rounds each entry once at the end
+        if (jac) fill_async(c, c.K, c.nnz, 0.0);
         if (func) fill_async(c, c.P, c.n_rows, 0.0);
         if (jac && func && !fp32) launch_colors<DIM, true, true, double>(c, a);
-        else if (jac && !fp32) launch_colors<DIM, true, false, double>(c, a);
-        else if (jac && fp32) launch_colors<DIM, true, false, float>(c, a);
+        else if (jac) launch_colors<DIM, true, false, double>(c, a);
+        if (jac && fp32)
+            hipLaunchKernelGGL(pgfk::k_round32, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((c.nnz + 255) / 256, 65536))),
+                               dim3(256), 0, c.stream, c.nnz, (const double*)c.K, c.K32);
         if (func && (fp32 || !jac)) launch_colors<DIM, false, true, double>(c, a);
         return;
     }

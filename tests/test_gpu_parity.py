@@ -162,6 +162,10 @@ def test_gather_layouts_bit_identical(dim):
                 ctx.assemble(flags, dt=0.05)
             ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FUNC | L.PGF_ASM_ALL_ROWS, dt=0.05)
             got = (ctx.matrix(), ctx.internal_force(), ctx.isv()[0])
+            # FP32 values: the FP64 sum rounded once, in every layout (prescribed
+            # rows are summed outside the LDS segments)
+            ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FP32 | L.PGF_ASM_ALL_ROWS, dt=0.05)
+            assert np.array_equal(ctx.matrix32(), got[0].astype(np.float32))
             print("dim %d slots %d node_rows %d" % (dim, slots, node_rows), flush=True)
             if ref is None:
                 ref = got
@@ -170,6 +174,11 @@ def test_gather_layouts_bit_identical(dim):
             else:
                 for g, r in zip(got, ref):
                     assert np.array_equal(g, r)
+    # coloured scheme: FP32 values are its FP64 sums rounded once
+    ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS | L.PGF_ASM_COLORED, dt=0.05)
+    k64 = ctx.matrix()
+    ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS | L.PGF_ASM_COLORED | L.PGF_ASM_FP32, dt=0.05)
+    assert np.array_equal(ctx.matrix32(), k64.astype(np.float32))
     with pytest.raises(RuntimeError):
         ctx.gather_layout = (3, 0)
     with pytest.raises(RuntimeError):
