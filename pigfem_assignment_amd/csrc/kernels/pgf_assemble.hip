@@ -188,6 +188,53 @@ __device__ inline void qp_geometry(const double (*X)[DIM], double tet_scale, int
         w = ws * tet_scale;
     }
 }
+// Row (la, da) of an affine QUAD4 (a parallelogram: x0 - x1 + x2 - x3 = 0
+// exactly, every element of a generated grid whose nodes were not nudged)
+// integrated in closed form.  With J constant, the 2x2 Gauss sum of
+// grad N_a (x) grad N_b is det J * inv M_ab inv^T with the reference moments
+// M_ab = sum_q dN_a/dxi(q) dN_b/dxi(q)^T:
+//   M00 = xi_a xi_b (1 + eta_a eta_b / 3) / 4,  M01 = xi_a eta_b / 4,
+//   M10 = eta_a xi_b / 4,  M11 = eta_a eta_b (1 + xi_a xi_b / 3) / 4.
+// The 2x2 rule integrates these products exactly (degree two per variable),
+// so the entries equal the quadrature's up to rounding; the row's internal
+// force is the row times the element displacements (linear elasticity).
+__device__ inline bool quad_affine(const double (*X)[2]) {
+    return (X[0][0] - X[1][0]) + (X[2][0] - X[3][0]) == 0.0 && (X[0][1] - X[1][1]) + (X[2][1] - X[3][1]) == 0.0;
+}
+template <int NR>
+__device__ inline void quad_affine_rows(const double (*X)[2], int la, const int* da, const Iso& m, double (*Kr)[8]) {
+    // constant Jacobian from the centre derivatives dN_n/dxi = (xi_n, eta_n) / 4
+    const double J00 = 0.25 * ((X[1][0] - X[0][0]) + (X[2][0] - X[3][0]));
+    const double J01 = 0.25 * ((X[1][1] - X[0][1]) + (X[2][1] - X[3][1]));
+    const double J10 = 0.25 * ((X[2][0] - X[1][0]) + (X[3][0] - X[0][0]));
+    const double J11 = 0.25 * ((X[2][1] - X[1][1]) + (X[3][1] - X[0][1]));
+    const double det = J00 * J11 - J01 * J10;
+    const double rd = 1.0 / det;
+    const double i00 = J11 * rd, i01 = -J01 * rd, i10 = -J10 * rd, i11 = J00 * rd;
+    const double xa = (la == 1 || la == 2) ? 1.0 : -1.0, ea = la >= 2 ? 1.0 : -1.0;
+    const double third = 1.0 / 3.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const double xb = (b == 1 || b == 2) ? 1.0 : -1.0, eb = b >= 2 ? 1.0 : -1.0;
+        const double M00 = 0.25 * xa * xb * (1.0 + ea * eb * third), M01 = 0.25 * xa * eb;
+        const double M10 = 0.25 * ea * xb, M11 = 0.25 * ea * eb * (1.0 + xa * xb * third);
+        // T = M inv^T, I = det inv T: I_ij = integral of g_a,i g_b,j
+        const double T00 = M00 * i00 + M01 * i01, T01 = M00 * i10 + M01 * i11;
+        const double T10 = M10 * i00 + M11 * i01, T11 = M10 * i10 + M11 * i11;
+        const double I00 = det * (i00 * T00 + i01 * T10), I01 = det * (i00 * T01 + i01 * T11);
+        const double I10 = det * (i10 * T00 + i11 * T10), I11 = det * (i10 * T01 + i11 * T11);
+        for (int h = 0; h < NR; ++h) {
+            if (da[h] == 0) {
+                Kr[h][b * 2 + 0] = m.a * I00 + m.c * I11;
+                Kr[h][b * 2 + 1] = m.b * I01 + m.c * I10;
+            } else {
+                Kr[h][b * 2 + 0] = m.b * I10 + m.c * I01;
+                Kr[h][b * 2 + 1] = m.a * I11 + m.c * I00;
+            }
+        }
+    }
+}
+
 template <int DIM>
 __device__ inline void uncut_geometry(const double (*X)[DIM], double tet_scale, double (*g)[4 * DIM], double* w) {
     constexpr int NQ = DIM == 2 ? 4 : 1;
@@ -861,9 +908,16 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                         for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
                 for (int h = 0; h < NR; ++h)
                     for (int j = 0; j < 4 * DIM; ++j) Kr[h][j] = 0.0;
+                const bool affine = DIM == 2 && quad_affine((const double(*)[2])X);
+                if (affine) {
+                    quad_affine_rows<NR>((const double(*)[2])X, la, da, D, (double(*)[8])Kr);
+                    if (DO_F)
+                        for (int h = 0; h < NR; ++h)
+                            for (int j = 0; j < 4 * DIM; ++j) Pc[h] += Kr[h][j] * u[j];
+                }
                 // one quadrature point at a time keeps the register footprint small
 #pragma unroll 1
-                for (int q = 0; q < NQ; ++q) {
+                for (int q = 0; q < (affine ? 0 : NQ); ++q) {
                     double g[4 * DIM], w;
                     qp_geometry<DIM>(X, a.tet_scale, q, g, w);
                     // the row node's gradient, selected without indexing the
