@@ -219,6 +219,15 @@ PGF_API int pgf_set_guess(pgf_ctx* ctx, const double* dx);
 PGF_API int pgf_batch_pcg(pgf_ctx* ctx, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs, const double* b,
                           double* x, double rtol, int32_t maxit, int32_t* iters, double* relres, double* ms,
                           int32_t* tune);
+/* The same solve as one persistent cooperative launch: every iteration of
+ * every pair runs inside a single kernel, grid-wide barriers between the
+ * phases (p update, SpMM, alpha, vector update, beta and convergence), no
+ * host round trip and no graph replay until all pairs have converged or hit
+ * maxit.  Same arithmetic in the same order: results equal pgf_batch_pcg's
+ * bit for bit.  Same arguments. */
+PGF_API int pgf_batch_pcg_persistent(pgf_ctx* ctx, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs,
+                                     const double* b, double* x, double rtol, int32_t maxit, int32_t* iters,
+                                     double* relres, double* ms, int32_t* tune);
 
 /* right-hand sides of prescribed-displacement load cases: b = -K_fp u_p
  * (src/SolverLinear.cpp:128-129 per load case); up host [nrhs][n_pres], b host [nrhs][n_free] */

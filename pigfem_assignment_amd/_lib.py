@@ -107,7 +107,7 @@ EXPORTS = [
     "pgf_net_create", "pgf_net_destroy", "pgf_net_last_error", "pgf_net_num_params", "pgf_net_get_params",
     "pgf_net_set_params", "pgf_net_forward", "pgf_net_backward", "pgf_net_add_sample", "pgf_net_num_samples",
     "pgf_net_train", "pgf_learn_attach", "pgf_learn_detach", "pgf_learn_commit", "pgf_learn_record", "pgf_learn_last",
-    "pgf_learn_features", "pgf_batch_pcg", "pgf_rhs_prescribed", "pgf_reactions", "pgf_set_solution_dofs",
+    "pgf_learn_features", "pgf_batch_pcg", "pgf_batch_pcg_persistent", "pgf_rhs_prescribed", "pgf_reactions", "pgf_set_solution_dofs",
     "pgf_volume_average", "pgf_volume_average_noalloc", "pgfh_shape_sensitivity", "pgf_set_shape_sensitivity",
     "pgf_shape_pseudo_load", "pgfh_write_vtk", "pgfh_batch_create", "pgfh_batch_destroy", "pgfh_batch_desc",
     "pgfh_batch_offsets", "pgfh_batch_get_dofs",
@@ -149,6 +149,7 @@ def lib():
     L.pgf_net_train.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.POINTER(C.c_float)]
     L.pgf_batch_pcg.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.c_int32, dp, dp, C.c_double, C.c_int32,
                                 C.POINTER(C.c_int32), dp, dp, C.POINTER(C.c_int32)]
+    L.pgf_batch_pcg_persistent.argtypes = L.pgf_batch_pcg.argtypes
     L.pgf_learn_attach.argtypes = [C.c_void_p, C.c_void_p, dp, C.POINTER(C.c_float)]
     L.pgf_set_solution_dofs.argtypes = [C.c_void_p, dp, dp]
     L.pgf_volume_average.argtypes = [C.c_void_p, dp, C.c_int32, C.POINTER(C.c_int64), dp]

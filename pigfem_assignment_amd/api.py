@@ -366,9 +366,11 @@ class DeviceContext:
         return ms.value
 
     # ---- batched solves (pgf_batch_pcg) ----
-    def batch_pcg(self, sys_ptr, B, rtol=1e-10, maxit=100000, tune=(0, 0)):
+    def batch_pcg(self, sys_ptr, B, rtol=1e-10, maxit=100000, tune=(0, 0), persistent=False):
         """Solve K_ff X = B per system (free-row ranges sys_ptr) for the rows of
-        B [nrhs][n_free]; returns X, iterations [n_sys][nrhs], relres, ms, tune."""
+        B [nrhs][n_free]; returns X, iterations [n_sys][nrhs], relres, ms, tune.
+        persistent: the whole solve in one cooperative launch
+        (pgf_batch_pcg_persistent) instead of graph replays."""
         sp = np.ascontiguousarray(sys_ptr, np.int64)
         B = np.ascontiguousarray(B, np.float64)
         nrhs = B.shape[0]
@@ -378,7 +380,8 @@ class DeviceContext:
         rel = np.zeros((n_sys, nrhs))
         ms = C.c_double()
         tn = (C.c_int32 * 2)(int(tune[0]), int(tune[1]))
-        rc = self.lib.pgf_batch_pcg(self.h, n_sys, sp.ctypes.data_as(C.POINTER(C.c_int64)), nrhs, L.ptr(B, C.c_double),
+        fn = self.lib.pgf_batch_pcg_persistent if persistent else self.lib.pgf_batch_pcg
+        rc = fn(self.h, n_sys, sp.ctypes.data_as(C.POINTER(C.c_int64)), nrhs, L.ptr(B, C.c_double),
                                     L.ptr(X, C.c_double), rtol, maxit, its.ctypes.data_as(C.POINTER(C.c_int32)),
                                     L.ptr(rel, C.c_double), C.byref(ms), tn)
         if rc not in (0, -5):

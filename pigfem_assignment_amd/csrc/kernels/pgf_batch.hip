@@ -18,6 +18,7 @@
 //  * the tile size (rows per tile) and threads per row are chosen by a small
 //    autotuner that times a few iterations of each candidate on the batch.
 // Vectors are interleaved [n_free][NR].
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,9 +82,10 @@ __device__ inline void block_sum(double (&v)[NR], double* out) {
 }
 
 // p = z + beta p on the tile's rows, in place, for the pairs still running
+// (li: index into the launch list; the whole block calls it)
 template <int NR>
-__global__ __launch_bounds__(BT) void k_bpupd(BArgs a) {
-    const int64_t t = a.tile_list[blockIdx.x];
+__device__ inline void bpupd_tile(const BArgs& a, int64_t li) {
+    const int64_t t = a.tile_list[li];
     const int s = a.tile_sys[t];
     if (tile_done<NR>(a, s)) return;
     double beta[NR];
@@ -96,12 +98,16 @@ __global__ __launch_bounds__(BT) void k_bpupd(BArgs a) {
         for (int c = 0; c < NR; ++c)
             if (act[c]) a.pnew[r * NR + c] = a.z[r * NR + c] + beta[c] * a.pnew[r * NR + c];
 }
+template <int NR>
+__global__ __launch_bounds__(BT) void k_bpupd(BArgs a) {
+    bpupd_tile<NR>(a, blockIdx.x);
+}
 
 // q = K p over the row's columns (p gathered once per column for all NR
 // right-hand sides), partial p . q.  G threads per row.
 template <int NR, int G>
-__global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
-    const int64_t t = a.tile_list[blockIdx.x];
+__device__ inline void bspmm_tile(const BArgs& a, int64_t li) {
+    const int64_t t = a.tile_list[li];
     const int s = a.tile_sys[t];
     if (tile_done<NR>(a, s)) return;
     bool act[NR];
@@ -133,14 +139,17 @@ __global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
     if (threadIdx.x == 0)
         for (int c = 0; c < NR; ++c) a.tpart[t * NR + c] = tot[c];
 }
+template <int NR, int G>
+__global__ __launch_bounds__(BT) void k_bspmm(BArgs a) {
+    bspmm_tile<NR, G>(a, blockIdx.x);
+}
 
 // per (system, right-hand side) pair, one wavefront: pq = sum of the
 // system's tile partials (lane-strided, then a fixed butterfly), alpha = rz / pq
+// (k: the pair, one whole wavefront per call)
 template <int NR>
-__global__ __launch_bounds__(256) void k_breduce_pq(BArgs a) {
-    const int k = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64);
+__device__ inline void breduce_pq_pair(const BArgs& a, int k) {
     const int lane = threadIdx.x & 63;
-    if (k >= a.n_sys * NR) return;  // uniform per wavefront
     const int s = k / NR, c = k % NR;
     if (a.done[k]) return;
     double pq = 0.0;
@@ -152,11 +161,17 @@ __global__ __launch_bounds__(256) void k_breduce_pq(BArgs a) {
         sc[S_ALPHA] = pq != 0.0 ? sc[S_RZ] / pq : 0.0;
     }
 }
+template <int NR>
+__global__ __launch_bounds__(256) void k_breduce_pq(BArgs a) {
+    const int k = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    if (k >= a.n_sys * NR) return;  // uniform per wavefront
+    breduce_pq_pair<NR>(a, k);
+}
 
 // x += alpha p, r -= alpha q, z = dinv r; partial r.z and r.r
 template <int NR>
-__global__ __launch_bounds__(BT) void k_bupdate(BArgs a) {
-    const int64_t t = a.tile_list[blockIdx.x];
+__device__ inline void bupdate_tile(const BArgs& a, int64_t li) {
+    const int64_t t = a.tile_list[li];
     const int s = a.tile_sys[t];
     if (tile_done<NR>(a, s)) return;
     double alpha[NR];
@@ -190,14 +205,16 @@ __global__ __launch_bounds__(BT) void k_bupdate(BArgs a) {
             a.tpart2[t * 2 * NR + NR + c] = trr[c];
         }
 }
+template <int NR>
+__global__ __launch_bounds__(BT) void k_bupdate(BArgs a) {
+    bupdate_tile<NR>(a, blockIdx.x);
+}
 
 // per pair, one wavefront: beta = rz_new / rz, convergence test rr <= tol2,
 // iteration count; every pair still running afterwards sets *active
 template <int NR>
-__global__ __launch_bounds__(256) void k_breduce_rz(BArgs a, int maxit) {
-    const int k = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64);
+__device__ inline void breduce_rz_pair(const BArgs& a, int k, int maxit, int* active) {
     const int lane = threadIdx.x & 63;
-    if (k >= a.n_sys * NR) return;  // uniform per wavefront
     if (a.done[k]) return;
     const int s = k / NR, c = k % NR;
     double rz = 0.0, rr = 0.0;
@@ -218,7 +235,48 @@ __global__ __launch_bounds__(256) void k_breduce_rz(BArgs a, int maxit) {
     if (rr <= sc[S_TOL2] || sc[S_IT] >= maxit || !(rr == rr)) {
         a.done[k] = 1;
     } else {
-        *a.active = 1;
+        *active = 1;
+    }
+}
+template <int NR>
+__global__ __launch_bounds__(256) void k_breduce_rz(BArgs a, int maxit) {
+    const int k = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    if (k >= a.n_sys * NR) return;  // uniform per wavefront
+    breduce_rz_pair<NR>(a, k, maxit, a.active);
+}
+
+// The whole solve in one cooperative launch: every block loops over the
+// iterations, takes launch-list entries blockIdx.x, blockIdx.x + gridDim.x, ...
+// in each tile phase and pairs wave by wave in each reduction phase, with a
+// grid-wide barrier between phases.  Same device functions as the kernels
+// above, so the same bits.  The running flag alternates between two slots:
+// iteration it raises active[it & 1] and clears active[(it + 1) & 1], which
+// every block read for the last time four barriers earlier.
+template <int NR, int G>
+__global__ __launch_bounds__(BT) void k_bpersist(BArgs a, int maxit, int* active) {
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int waves = (int)(gridDim.x * (BT / 64));
+    const int wave = (int)(blockIdx.x * (BT / 64) + threadIdx.x / 64);
+    const int pairs = a.n_sys * NR;
+    for (int it = 0; it <= maxit; ++it) {
+        for (int64_t li = blockIdx.x; li < a.n_launch; li += gridDim.x) bpupd_tile<NR>(a, li);
+        grid.sync();
+        for (int64_t li = blockIdx.x; li < a.n_launch; li += gridDim.x) bspmm_tile<NR, G>(a, li);
+        grid.sync();
+        for (int k = wave; k < pairs; k += waves) breduce_pq_pair<NR>(a, k);
+        grid.sync();
+        for (int64_t li = blockIdx.x; li < a.n_launch; li += gridDim.x) bupdate_tile<NR>(a, li);
+        grid.sync();
+        if (blockIdx.x == 0 && threadIdx.x == 0) active[(it + 1) & 1] = 0;
+        for (int k = wave; k < pairs; k += waves) breduce_rz_pair<NR>(a, k, maxit, &active[it & 1]);
+        grid.sync();
+        // one coherent read per block (an atomic at L2, not an L1 line that
+        // another block's earlier value may still occupy), shared with the block
+        __shared__ int run;
+        if (threadIdx.x == 0) run = atomicAdd(&active[it & 1], 0);
+        __syncthreads();
+        if (run == 0) break;  // the same value in every block
+        __syncthreads();      // every thread has read run before thread 0 rewrites it
     }
 }
 
@@ -384,8 +442,16 @@ void enqueue_iteration_g(int G, BArgs& a, hipStream_t st, int maxit) {
 }
 
 template <int NR>
+const void* persist_kernel(int G) {
+    if (G == 4) return reinterpret_cast<const void*>(&k_bpersist<NR, 4>);
+    if (G == 8) return reinterpret_cast<const void*>(&k_bpersist<NR, 8>);
+    return reinterpret_cast<const void*>(&k_bpersist<NR, 16>);
+}
+
+template <int NR>
 int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b, double* x, double rtol, int maxit,
-              int32_t* iters, double* relres, double* ms_out, int32_t* tuned, int G_req, int rows_req) {
+              int32_t* iters, double* relres, double* ms_out, int32_t* tuned, int G_req, int rows_req,
+              bool persistent) {
     const int n_sys = (int)sys_ptr.size() - 1;
     const int64_t nf = c.n_free;
     // autotuner candidates: (threads per row, rows per tile)
@@ -478,6 +544,38 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
     if (tuned) { tuned[0] = bestG; tuned[1] = bestR; }
     set_tiles(bestR);
     init();
+    if (persistent) {
+        // ---- the whole solve in one cooperative launch ----
+        int* act2 = balloc<int>(2);
+        BCHK(hipMemsetAsync(act2, 0, 2 * sizeof(int), c.stream));
+        int dev = 0, per_cu = 0;
+        BCHK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        BCHK(hipGetDeviceProperties(&prop, dev));
+        const void* kfn = persist_kernel<NR>(bestG);
+        BCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BT, 0));
+        const int64_t grid = std::min<int64_t>((int64_t)per_cu * prop.multiProcessorCount, std::max<int64_t>(a.n_launch, 1));
+        if (grid < 1) {
+            (void)hipFree(act2);
+            (void)hipHostFree(h_active);
+            throw std::string("the persistent batch kernel cannot be resident on this device");
+        }
+        int mi = maxit;
+        void* args[] = {(void*)&a, (void*)&mi, (void*)&act2};
+        hipEvent_t t0, t1;
+        BCHK(hipEventCreate(&t0));
+        BCHK(hipEventCreate(&t1));
+        BCHK(hipEventRecord(t0, c.stream));
+        BCHK(hipLaunchCooperativeKernel(kfn, dim3((unsigned)grid), dim3(BT), args, 0, c.stream));
+        BCHK(hipEventRecord(t1, c.stream));
+        BCHK(hipEventSynchronize(t1));
+        float ms;
+        BCHK(hipEventElapsedTime(&ms, t0, t1));
+        if (ms_out) *ms_out = ms;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        (void)hipFree(act2);
+    } else {
     // ---- capture CHUNK iterations in one graph, replay until done ----
     const int CHUNK = 16;
     int* h_done = nullptr;
@@ -537,6 +635,7 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
     (void)hipHostFree(h_done);
+    }
     // results
     std::vector<double> xi((size_t)nf * NR), sc((size_t)n_sys * NR * S_N);
     BCHK(hipMemcpy(xi.data(), a.x, sizeof(double) * xi.size(), hipMemcpyDeviceToHost));
@@ -615,8 +714,9 @@ PGF_API int pgf_reactions(pgf_ctx* h, int32_t nrhs, const double* uf, const doub
 // start, each (system, rhs) pair to ||r|| <= rtol ||b||.  b and x are host
 // arrays [nrhs][n_free].  tune[0..1] in: threads per row and rows per tile
 // (0: autotune); out: the choice used.
-PGF_API int pgf_batch_pcg(pgf_ctx* h, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs, const double* b, double* x,
-                          double rtol, int32_t maxit, int32_t* iters, double* relres, double* ms, int32_t* tune) {
+static int batch_pcg(pgf_ctx* h, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs, const double* b, double* x,
+                     double rtol, int32_t maxit, int32_t* iters, double* relres, double* ms, int32_t* tune,
+                     bool persistent) {
     if (!h || n_sys <= 0 || !sys_ptr || !b || !x || nrhs < 1 || nrhs > 4 || maxit < 1) return PGF_EARG;
     pgfd::Ctx* c = pgfd::ctx_of(h);
     try {
@@ -635,15 +735,27 @@ PGF_API int pgf_batch_pcg(pgf_ctx* h, int32_t n_sys, const int64_t* sys_ptr, int
         const int Greq = tune ? tune[0] : 0, Rreq = tune ? tune[1] : 0;
         if (Greq && Greq != 4 && Greq != 8 && Greq != 16) throw std::string("threads per row must be 4, 8 or 16");
         int rc;
-        if (nrhs == 1) rc = run_batch<1>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq);
-        else if (nrhs == 2) rc = run_batch<2>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq);
-        else if (nrhs == 3) rc = run_batch<3>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq);
-        else rc = run_batch<4>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq);
+        const bool pe = persistent;
+        if (nrhs == 1) rc = run_batch<1>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq, pe);
+        else if (nrhs == 2) rc = run_batch<2>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq, pe);
+        else if (nrhs == 3) rc = run_batch<3>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq, pe);
+        else rc = run_batch<4>(*c, sp, b, x, rtol, maxit, iters, relres, ms, tune, Greq, Rreq, pe);
         return rc;
     } catch (const std::string& e) {
         c->err = e;
         return e.rfind("HIP", 0) == 0 ? PGF_EDEVICE : PGF_EARG;
     }
+}
+
+PGF_API int pgf_batch_pcg(pgf_ctx* h, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs, const double* b, double* x,
+                          double rtol, int32_t maxit, int32_t* iters, double* relres, double* ms, int32_t* tune) {
+    return batch_pcg(h, n_sys, sys_ptr, nrhs, b, x, rtol, maxit, iters, relres, ms, tune, false);
+}
+
+PGF_API int pgf_batch_pcg_persistent(pgf_ctx* h, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs, const double* b,
+                                     double* x, double rtol, int32_t maxit, int32_t* iters, double* relres, double* ms,
+                                     int32_t* tune) {
+    return batch_pcg(h, n_sys, sys_ptr, nrhs, b, x, rtol, maxit, iters, relres, ms, tune, true);
 }
 
 }  // extern "C"

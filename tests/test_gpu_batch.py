@@ -72,6 +72,28 @@ def test_batch_solves_match_single_solves(rves):
     assert np.max(np.abs(X2 - X)) <= 1e-9 * np.max(np.abs(X))
 
 
+@pytest.mark.parametrize("tune", [(4, 128), (8, 512), (16, 2048)])
+def test_persistent_batch_equals_graph_batch(rves, tune):
+    """The whole segmented PCG in one cooperative launch gives the graph
+    replays' solutions and iteration counts bit for bit (same arithmetic in
+    the same order), for every (threads per row, rows per tile) choice."""
+    B = batch.BatchProblem([p for p, _ in rves])
+    ctx = DeviceContext(B)
+    ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS)
+    RHS = ctx.rhs_prescribed(B.case_values())
+    Xg, itg, relg, msg, _ = ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-12, tune=tune)
+    Xp, itp, relp, msp, tp = ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-12, tune=tune, persistent=True)
+    print("tune %s: graph %.2f ms, persistent %.2f ms, iterations max %d" % (tune, msg, msp, itg.max()), flush=True)
+    assert tp == tune
+    assert np.array_equal(itp, itg)
+    assert np.array_equal(Xp, Xg)
+    assert np.array_equal(relp, relg)
+    # a cap on the iterations stops the persistent kernel too
+    Xc, itc, relc, _, _ = ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-14, maxit=25, tune=tune, persistent=True)
+    Xc2, itc2, _, _, _ = ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-14, maxit=25, tune=tune)
+    assert np.all(itc == 25) and np.array_equal(itc, itc2) and np.array_equal(Xc, Xc2)
+
+
 def test_effective_stiffness_closed_forms():
     Em, num = configs.MATRIX
     homog = batch.rve_problem(32, 1, inclusions=np.zeros((0, 5)))[0]
