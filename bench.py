@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--c3-nmin", type=int, default=64)
     ap.add_argument("--c3-nmax", type=int, default=512)
     ap.add_argument("--c3-slots", type=int, default=1, help="inclusion radii per RVE with direct sensitivities")
+    ap.add_argument("--c3-persistent", action="store_true",
+                    help="also time the load-case solve as one persistent cooperative launch (small batches only)")
     ap.add_argument("--setup", choices=["device", "host"], default="device",
                     help="where the IGFEM preprocessing (detection, enrichment, integration tables) runs")
     args = ap.parse_args()
@@ -156,6 +158,8 @@ def main():
     extra["assembly_fp64f_ms"] = ms_f64
     extra["spmv_fp64_ms"] = ms_spmv
     extra["spmv_fp32_ms"] = ms_spmv32
+    extra["spmv_fp64_roofline_frac"] = extra["spmv_fp64_GBs"] / HBM_PEAK_GBS
+    extra["spmv_fp32_roofline_frac"] = extra["spmv_fp32_GBs"] / HBM_PEAK_GBS
 
     if not args.no_solve:
         # time to a 1e-8 FP64 residual for the load step
@@ -450,8 +454,9 @@ def bench_c2(args):
         gbs = (vb * nnz_ff + 4 * (nf + 1) + 16 * nf) / (t * 1e-3) / 1e9
         extra["spmv_fp%d_ms" % prec] = t
         extra["spmv_fp%d_GBs" % prec] = gbs
-        print("# C2 SpMV K_ff (%d nonzeros) %s: %.3f ms, %.0f GB/s" % (
-            nnz_ff, "FP64" if prec == 64 else "FP32 values / FP64 vectors", t, gbs), flush=True)
+        extra["spmv_fp%d_roofline_frac" % prec] = gbs / HBM_PEAK_GBS
+        print("# C2 SpMV K_ff (%d nonzeros) %s: %.3f ms, %.0f GB/s (%.2f of 8 TB/s)" % (
+            nnz_ff, "FP64" if prec == 64 else "FP32 values / FP64 vectors", t, gbs, gbs / HBM_PEAK_GBS), flush=True)
     cpu = None
     if not args.no_cpu:
         model, avail = cpu_info()
@@ -527,11 +532,24 @@ def bench_c3(args):
     F = batch.macro_strain_rhs(ctx, B)
     ctx.assemble(L.PGF_ASM_JAC)
     X, its, rel, ms_solve, tune = ctx.batch_pcg(B.sys_ptr, F, rtol=1e-8)
+    print("# C3 batched PCG (graph replays): %.1f ms" % ms_solve, flush=True)
+    ms_persist, same = None, None
+    if args.c3_persistent:
+        # the same solve as one persistent cooperative launch (bit-identical)
+        try:
+            Xp, itp, _, ms_persist, _ = ctx.batch_pcg(B.sys_ptr, F, rtol=1e-8, tune=tune, persistent=True)
+            same = bool(np.array_equal(Xp, X) and np.array_equal(itp, its))
+            del Xp
+            print("# C3 batched PCG as one persistent launch: %.1f ms (graph replays %.1f ms), identical results %s"
+                  % (ms_persist, ms_solve, same), flush=True)
+        except RuntimeError as e:
+            print("# C3 persistent launch refused: %s" % e, flush=True)
     Ce = batch.homogenized_stiffness(ctx, B, X, B.elem_ptr)
     print("# C3 assembly %.2f ms (%.3e elem/s); batched PCG 3 cases x %d RVEs: %.1f ms, iterations %d..%d, "
           "tuned (threads/row, rows/tile) %s" % (ms_asm, B.n_elem / (ms_asm * 1e-3), len(parts), ms_solve,
                                                  its.min(), its.max(), tune), flush=True)
-    extra = {"batched_solve_ms": ms_solve, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
+    extra = {"batched_solve_ms": ms_solve, "batched_solve_persistent_ms": ms_persist,
+             "persistent_equals_graph": same, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
              "iterations_total": int(its.sum()), "autotuned": list(tune), "gather_layout": slots,
              "setup_s": setup_s, "setup_igfem_on": args.setup, "setup_igfem_phase_ms": phases.tolist(),
              "C_eff_mean": Ce.mean(axis=0).tolist()}

@@ -224,7 +224,8 @@ PGF_API int pgf_batch_pcg(pgf_ctx* ctx, int32_t n_sys, const int64_t* sys_ptr, i
  * phases (p update, SpMM, alpha, vector update, beta and convergence), no
  * host round trip and no graph replay until all pairs have converged or hit
  * maxit.  Same arithmetic in the same order: results equal pgf_batch_pcg's
- * bit for bit.  Same arguments. */
+ * bit for bit.  Same arguments.  Holds at most 256 row tiles, one block per
+ * CU (PGF_EUNSUPPORTED beyond): the size it has been run and checked at. */
 PGF_API int pgf_batch_pcg_persistent(pgf_ctx* ctx, int32_t n_sys, const int64_t* sys_ptr, int32_t nrhs,
                                      const double* b, double* x, double rtol, int32_t maxit, int32_t* iters,
                                      double* relres, double* ms, int32_t* tune);

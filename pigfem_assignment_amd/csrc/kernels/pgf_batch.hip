@@ -441,6 +441,8 @@ void enqueue_iteration_g(int G, BArgs& a, hipStream_t st, int maxit) {
     else enqueue_iteration<NR, 16>(a, st, maxit);
 }
 
+constexpr int64_t PERSIST_MAX_TILES = 256;
+
 template <int NR>
 const void* persist_kernel(int G) {
     if (G == 4) return reinterpret_cast<const void*>(&k_bpersist<NR, 4>);
@@ -554,6 +556,15 @@ int run_batch(pgfd::Ctx& c, const std::vector<int64_t>& sys_ptr, const double* b
         BCHK(hipGetDeviceProperties(&prop, dev));
         const void* kfn = persist_kernel<NR>(bestG);
         BCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BT, 0));
+        // one block per tile, at most one per CU: the size this path has been
+        // run and checked at; larger batches are refused (their runs did not
+        // finish within minutes where the graph replays took 15.7 s)
+        if (a.n_launch > PERSIST_MAX_TILES || a.n_launch > (int64_t)prop.multiProcessorCount) {
+            (void)hipFree(act2);
+            (void)hipHostFree(h_active);
+            throw std::string("persistent batch launch holds at most ") + std::to_string(PERSIST_MAX_TILES) +
+                " row tiles (one per CU); this batch has " + std::to_string(a.n_launch) + ": use pgf_batch_pcg";
+        }
         const int64_t grid = std::min<int64_t>((int64_t)per_cu * prop.multiProcessorCount, std::max<int64_t>(a.n_launch, 1));
         if (grid < 1) {
             (void)hipFree(act2);
@@ -743,6 +754,7 @@ static int batch_pcg(pgf_ctx* h, int32_t n_sys, const int64_t* sys_ptr, int32_t 
         return rc;
     } catch (const std::string& e) {
         c->err = e;
+        if (e.rfind("persistent", 0) == 0) return PGF_EUNSUPPORTED;
         return e.rfind("HIP", 0) == 0 ? PGF_EDEVICE : PGF_EARG;
     }
 }

@@ -94,6 +94,18 @@ def test_persistent_batch_equals_graph_batch(rves, tune):
     assert np.all(itc == 25) and np.array_equal(itc, itc2) and np.array_equal(Xc, Xc2)
 
 
+def test_persistent_batch_refuses_large_batches():
+    """More row tiles than the persistent launch holds: a status code and a
+    message, no launch (the graph path serves such batches)."""
+    B = batch.BatchProblem([batch.rve_problem(N, seed)[0] for N, seed in [(96, 21), (128, 22)]])
+    ctx = DeviceContext(B)
+    ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_ALL_ROWS)
+    RHS = ctx.rhs_prescribed(B.case_values())
+    # (4 threads per row, 128 rows per tile) gives more than 256 tiles here
+    with pytest.raises(RuntimeError, match="at most 256 row tiles"):
+        ctx.batch_pcg(B.sys_ptr, RHS, rtol=1e-8, tune=(4, 128), persistent=True)
+
+
 def test_effective_stiffness_closed_forms():
     Em, num = configs.MATRIX
     homog = batch.rve_problem(32, 1, inclusions=np.zeros((0, 5)))[0]
