@@ -557,45 +557,30 @@ __global__ __launch_bounds__(cut_threads<DIM>()) void k_cut(AsmArgs a, int64_t b
         }
 }
 
-// Gather scheme, pass 1: every cut element's K_e / P_e into the dense scratch.
-template <int DIM, bool DO_K, bool DO_F>
-__global__ __launch_bounds__(cut_threads<DIM>()) void k_cut_elem(AsmArgs a) {
-    constexpr int MM = MAX_NT * DIM;
-    const int64_t cl = xcd_logical(blockIdx.x, gridDim.x);
-    if (cl >= a.n_cut) return;  // padding block of the XCD mapping
-    const int c = (int)cl;
-    const int nt = a.ce_nt[c], M = nt * DIM, tid = threadIdx.x;
-    __shared__ double s_P[MM];
-    __shared__ int s_node[MAX_NT], s_dof[MM];
-    double* ke = a.ke + a.ke_ptr[c];
-    cut_compute<DIM, DO_K, DO_F>(a, c, nt, s_P, s_node, s_dof, [&](int i, int j, double v) { ke[i * M + j] = v; });
-    if (DO_F) {
-        double* pe = a.pe + a.pe_ptr[c];
-        for (int i = tid; i < M; i += cut_threads<DIM>()) pe[i] = s_P[i];
-    }
-}
-
 // ---------------------------------------------------------------------------
-// 3D cut-element matrices on the FP64 matrix cores.
+// Cut-element matrices on the FP64 matrix cores.
 //
-// The element matrix of a cut tet (src/Assembler.cpp:392-524: volumetric
+// The element matrix of a cut element (src/Assembler.cpp:392-524: volumetric
 // sub-element points, then cohesive points) is one small dense product
 //     K_e = A^T C,
 // with the rows of A and C stacked over the element's integration points:
-//   * volumetric point q (7 rows): with the isotropic D = lam m m^T + mu D0,
-//     A = (m^T B_q; B_q) and C = diag(w lam, 2 w mu (x3), w mu (x3)) A, so
-//     A^T C = w B^T D B (SmallStrainKFast, src/Utilities.cpp:196-296);
-//   * cohesive point p (3 rows, jump component d):  A = L_p,  C = w_p T_p L_p
+//   * volumetric point q: with the isotropic D = lam m m^T + mu D0 (D0 =
+//     diag(2, 2, 1) in 2D, diag(2, 2, 2, 1, 1, 1) in 3D), A = (m^T B_q; B_q)
+//     and C = diag(w lam, 2 w mu (normal rows), w mu (shear rows)) A, so
+//     A^T C = w B^T D B (SmallStrainKFast, src/Utilities.cpp:196-296); 4 rows
+//     in 2D, 7 in 3D;
+//   * cohesive point p (DIM rows, jump component d):  A = L_p,  C = w_p T_p L_p
 //     with L_p (u) = sum_n N_n (u_upper,n - u_lower,n) and T_p = R^T D_c R
 //     (KernelCohesive / cohesiveKFast, src/AssemblerSmallStrainNonlinear.cpp:73-91,
 //     src/Utilities.cpp:480-508: the +-N_i N_j R^T D R blocks).
 // One wavefront per cut element, four per block.  Each lane forms the A and C
 // operand values it feeds to v_mfma_f64_16x16x4f64 straight from the point data
 // staged in LDS (gradients, weights, isotropic D, R^T D_c R, signed surface
-// shape functions), so no operand matrix is materialised; K_e (up to 36 x 36,
-// 2 x 2 or 3 x 3 output tiles of 16 x 16) accumulates in registers and is
-// written once to the cut-element scratch the gather pass reads.  P_e and the
-// cohesive law (trial delta_max and damage flags) are as in cut_compute.
+// shape functions), so no operand matrix is materialised; K_e (up to 16 x 16
+// in 2D, one output tile; up to 36 x 36 in 3D, 2 x 2 or 3 x 3 tiles of
+// 16 x 16) accumulates in registers and is written once to the cut-element
+// scratch the gather pass reads.  P_e and the cohesive law (trial delta_max
+// and damage flags) are as in cut_compute.
 //
 // v_mfma_f64_16x16x4f64 lane layout (wave64): A operand lane l holds
 // A(row l % 16, k l / 16); B operand lane l holds B(k l / 16, col l % 16); the
@@ -603,89 +588,106 @@ __global__ __launch_bounds__(cut_threads<DIM>()) void k_cut_elem(AsmArgs a) {
 // interleaved across the four lane groups, unlike the fp32 16x16x4 form).
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int MF_WAVES = 4;                    // cut elements per block
-constexpr int MF_NV = 16;                      // volumetric points staged (max_vpt(3))
-constexpr int MF_MM = MAX_NT * 3;              // 36
+constexpr int MF_WAVES = 4;  // cut elements per block
 
+template <int DIM>
 struct MfWave {
-    double g[MF_NV * MF_MM];      // slot gradients per point
-    double w[MF_NV];
-    double Dabc[MF_NV * 3];       // isotropic D: a (normal), b (off-diagonal), c (shear)
-    double wl[MF_NV], wm[MF_NV];  // w lam and w mu of every point (lam = b, mu = c)
-    double sig[MF_NV * 6];
-    double u[MF_MM];
-    double T[MAX_CPT * 9];        // w_p R^T D_c R
-    double cs[MAX_CPT * MAX_NT];  // signed surface shape function of every slot (upper +N, lower -N)
-    double ctR[MAX_CPT * 3];      // w_p R^T t
+    static constexpr int NV = max_vpt(DIM);  // volumetric points staged
+    static constexpr int MM = MAX_NT * DIM;
+    static constexpr int S = DIM == 2 ? 3 : 6;
+    double g[NV * MM];            // slot gradients per point
+    double w[NV];
+    double Dabc[NV * 3];          // isotropic D: a (normal), b (off-diagonal), c (shear)
+    double wl[NV], wm[NV];        // w lam and w mu of every point (lam = b, mu = c)
+    double sig[NV * S];
+    double u[MM];
+    double T[MAX_CPT * DIM * DIM];  // w_p R^T D_c R
+    double cs[MAX_CPT * MAX_NT];    // signed surface shape function of every slot (upper +N, lower -N)
+    double ctR[MAX_CPT * DIM];      // w_p R^T t
     int node[MAX_NT];
     int nv, nc, nt;
 };
 
-// Operand rows of a volumetric point q for the isotropic D = lam m m^T + mu D0,
-// D0 = diag(2, 2, 2, 1, 1, 1): B^T D B = lam (m^T B)^T (m^T B) + mu B^T D0 B,
-// so with the seven rows
+// Operand rows of a volumetric point q for the isotropic D = lam m m^T + mu D0:
+// B^T D B = lam (m^T B)^T (m^T B) + mu B^T D0 B, so with the rows
 //   s = 0: (m^T B)(a, da) = g_a[da]                      weight w lam
-//   s = 1..3: normal Voigt row c = s - 1 of B            weight 2 w mu
-//   s = 4..6: shear Voigt rows yz, xz, xy of B           weight w mu
-// C is the weight times A, and an entry of A is one gradient component or 0.
+//   3D: s = 1..3 normal Voigt rows, s = 4..6 shear rows yz, xz, xy
+//   2D: s = 1..2 normal Voigt rows xx, yy, s = 3 the shear row xy
+// (normal rows weight 2 w mu, shear rows w mu) C is the weight times A, and
+// an entry of A is one gradient component or 0.
 // A(kk, (a, da)) for row s: the gradient component it selects, or -1 for 0.
+template <int DIM>
 __device__ inline int vrow_comp(int s, int da) {
     if (s == 0) return da;
+    if (DIM == 2) {
+        if (s <= 2) return s - 1 == da ? da : -1;
+        return 1 - da;  // xy: u_x <- g_y, u_y <- g_x
+    }
     if (s <= 3) return s - 1 == da ? da : -1;
     if (s == 4) return da == 0 ? -1 : 3 - da;  // yz: u_y <- g_z, u_z <- g_y
     if (s == 5) return da == 1 ? -1 : 2 - da;  // xz: u_x <- g_z, u_z <- g_x
     return da == 2 ? -1 : 1 - da;              // xy: u_x <- g_y, u_y <- g_x
 }
+template <int DIM>
+constexpr int vrows() { return DIM == 2 ? 4 : 7; }
 
-// K_e = A^T C over the stacked rows: 7 per volumetric point, then 3 per
+// K_e = A^T C over the stacked rows: vrows per volumetric point, then DIM per
 // cohesive point (A = d-th jump row, C = T_p rows times the signed shape
 // functions).  Each lane evaluates, per k-step, the A entries of its column
 // in every tile (NT values) and scales them into C for volumetric rows.
-template <int NT>
-__device__ inline void mfma_ke(const MfWave& W, int M, int lane, double* ke) {
+template <int DIM, int NT>
+__device__ inline void mfma_ke(const MfWave<DIM>& W, int M, int lane, double* ke) {
+    constexpr int MM = MfWave<DIM>::MM, R = vrows<DIM>();
     v4d acc[NT][NT];
 #pragma unroll
     for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
         for (int tj = 0; tj < NT; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
-    const int KV = 7 * W.nv, KD = KV + 3 * W.nc;
+    const int KV = R * W.nv, KD = KV + DIM * W.nc;
     const int r16 = lane & 15, k4 = lane >> 4;
-    // this lane's column of every tile: slot a, component da
-    int ga[NT], da[NT];
+    // this lane's column of every tile: slot sl, component da
+    int sl[NT], da[NT];
     bool ok[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int i = 16 * t + r16;
         ok[t] = i < M;
-        const int a = i / 3;
-        da[t] = i - 3 * a;
-        ga[t] = a * 3;
+        sl[t] = i / DIM;
+        da[t] = i - DIM * sl[t];
     }
     for (int k0 = 0; k0 < KD; k0 += 4) {
         const int kk = k0 + k4;
         double av[NT], cv[NT];
         if (kk < KV) {
-            const int q = kk / 7, s = kk - 7 * q;
-            const double wt = s == 0 ? W.wl[q] : (s <= 3 ? 2.0 * W.wm[q] : W.wm[q]);
-            const double* gq = &W.g[q * MF_MM];
+            const int q = kk / R, s = kk - R * q;
+            const double wt = s == 0 ? W.wl[q] : (s <= DIM ? 2.0 * W.wm[q] : W.wm[q]);
+            const double* gq = &W.g[q * MM];
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const int cmp = vrow_comp(s, da[t]);
-                av[t] = ok[t] && cmp >= 0 ? gq[ga[t] + cmp] : 0.0;
+                const int cmp = vrow_comp<DIM>(s, da[t]);
+                av[t] = ok[t] && cmp >= 0 ? gq[sl[t] * DIM + cmp] : 0.0;
                 cv[t] = wt * av[t];
             }
         } else if (kk < KD) {
-            const int kc = kk - KV, p = kc / 3, d = kc - 3 * p;
+            const int kc = kk - KV, p = kc / DIM, d = kc - DIM * p;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const double sa = ok[t] ? W.cs[p * MAX_NT + ga[t] / 3] : 0.0;
+                const double sa = ok[t] ? W.cs[p * MAX_NT + sl[t]] : 0.0;
                 av[t] = d == da[t] ? sa : 0.0;
-                cv[t] = W.T[p * 9 + d * 3 + da[t]] * sa;
+                cv[t] = W.T[p * DIM * DIM + d * DIM + da[t]] * sa;
             }
         } else {
 #pragma unroll
             for (int t = 0; t < NT; ++t) av[t] = cv[t] = 0.0;
         }
+        // One output tile: each product accumulates into the tile the previous
+        // product (about twenty instructions earlier) is still writing, and the
+        // compiler inserts no wait for an exactly overlapping accumulator.  On
+        // gfx950 that gave wrong K_e that changed from run to run; 16 wait
+        // states before each product (placed after the operands are formed)
+        // give stable results within the oracle's bar.  With 2 x 2 or 3 x 3
+        // tiles the other tiles' products separate two uses of one tile.
+        if constexpr (NT == 1) asm volatile("s_nop 7\n\ts_nop 7" : "+v"(av[0]), "+v"(cv[0]));
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
@@ -707,26 +709,27 @@ __device__ inline void mfma_ke(const MfWave& W, int M, int lane, double* ke) {
 // accumulators in ordinary VGPRs (128 in all, no spills) instead of 131 VGPRs
 // plus 72 accumulation registers at two waves per SIMD; the staging loads of
 // one element overlap the products of the others.
-template <bool DO_K, bool DO_F>
+template <int DIM, bool DO_K, bool DO_F>
 __global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a) {
-    __shared__ MfWave s_w[MF_WAVES];
+    constexpr int MM = MfWave<DIM>::MM, S = MfWave<DIM>::S, SS = S * S;
+    __shared__ MfWave<DIM> s_w[MF_WAVES];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
     const int64_t cl = blk * MF_WAVES + wave;
     const bool live = cl < a.n_cut;  // padding waves keep to the barriers
-    MfWave& W = s_w[wave];
+    MfWave<DIM>& W = s_w[wave];
     const int c = live ? (int)cl : 0;
-    const int nt = live ? a.ce_nt[c] : 0, M = 3 * nt;
+    const int nt = live ? a.ce_nt[c] : 0, M = DIM * nt;
     const int64_t v0 = live ? a.vpt_ptr[c] : 0, c0 = live ? a.cpt_ptr[c] : 0;
     const int nv = live ? (int)(a.vpt_ptr[c + 1] - v0) : 0;
     const int nc = live ? (int)(a.cpt_ptr[c + 1] - c0) : 0;
     if (lane == 0) { W.nv = nv; W.nc = nc; W.nt = nt; }
     if (lane < nt) W.node[lane] = a.ce_nodes[c * 16 + lane];
     __syncthreads();
-    for (int i = lane; i < M; i += WAVE) W.u[i] = a.U[(i % 3) * a.n_all + W.node[i / 3]];
+    for (int i = lane; i < M; i += WAVE) W.u[i] = a.U[(i % DIM) * a.n_all + W.node[i / DIM]];
     for (int k = lane; k < nv * M; k += WAVE) {
         const int q = k / M;
-        W.g[q * MF_MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
+        W.g[q * MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
     }
     if (lane < nv) {
         const double w = a.vpt_w[v0 + lane];
@@ -734,23 +737,17 @@ __global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a)
         const double* D = a.Dmat + a.vpt_mat[v0 + lane] * 36;
         W.Dabc[3 * lane] = D[0];
         W.Dabc[3 * lane + 1] = D[1];
-        W.Dabc[3 * lane + 2] = D[35];
+        W.Dabc[3 * lane + 2] = D[SS - 1];
         W.wl[lane] = w * D[1];
-        W.wm[lane] = w * D[35];
+        W.wm[lane] = w * D[SS - 1];
     }
     for (int k = lane; k < nc * MAX_NT; k += WAVE) W.cs[k] = 0.0;
     __syncthreads();
     if (DO_F && lane < nv) {
-        double eps[6];
-        strain_of<3>(nt, &W.g[lane * MF_MM], W.u, eps);
-        const double aa = W.Dabc[3 * lane], bb = W.Dabc[3 * lane + 1], cc = W.Dabc[3 * lane + 2];
-        double* s = &W.sig[lane * 6];
-        s[0] = aa * eps[0] + bb * eps[1] + bb * eps[2];
-        s[1] = bb * eps[0] + aa * eps[1] + bb * eps[2];
-        s[2] = bb * eps[0] + bb * eps[1] + aa * eps[2];
-        s[3] = cc * eps[3];
-        s[4] = cc * eps[4];
-        s[5] = cc * eps[5];
+        double eps[S];
+        strain_of<DIM>(nt, &W.g[lane * MM], W.u, eps);
+        const Iso m{W.Dabc[3 * lane], W.Dabc[3 * lane + 1], W.Dabc[3 * lane + 2]};
+        stress_of<DIM>(m, eps, &W.sig[lane * S]);
     }
     if (lane < nc) {
         const int64_t q = c0 + lane;
@@ -761,52 +758,53 @@ __global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a)
             if (n >= nh) break;
             const double Nn = a.cpt_N[q * 4 + n];
             const int lo = a.cpt_nodes[q * 8 + n], hi = a.cpt_nodes[q * 8 + nh + n];
-            for (int d = 0; d < 3; ++d) dl[d] += Nn * (W.u[hi * 3 + d] - W.u[lo * 3 + d]);
+            for (int d = 0; d < DIM; ++d) dl[d] += Nn * (W.u[hi * DIM + d] - W.u[lo * DIM + d]);
             W.cs[lane * MAX_NT + lo] = -Nn;
             W.cs[lane * MAX_NT + hi] = Nn;
         }
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) delta[i] += R[i * 3 + j] * dl[j];
+        for (int i = 0; i < DIM; ++i)
+            for (int j = 0; j < DIM; ++j) delta[i] += R[i * DIM + j] * dl[j];
         double trac[3], Dc[9], dnew;
         uint8_t fl;
-        coh_law<3>(a.laws[a.cpt_law[q]], delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
+        coh_law<DIM>(a.laws[a.cpt_law[q]], delta, a.dmax_conv[q], a.dt, trac, Dc, dnew, fl);
         a.dmax_trial[q] = dnew;
         a.flags[q] = fl;
         const double wq = a.cpt_w[q];
         double RtD[9];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
+        for (int i = 0; i < DIM; ++i)
+            for (int j = 0; j < DIM; ++j) {
                 double s = 0.0;
-                for (int k = 0; k < 3; ++k) s += R[k * 3 + i] * Dc[k * 3 + j];
-                RtD[i * 3 + j] = s;
+                for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * Dc[k * DIM + j];
+                RtD[i * DIM + j] = s;
             }
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
+        for (int i = 0; i < DIM; ++i)
+            for (int j = 0; j < DIM; ++j) {
                 double s = 0.0;
-                for (int k = 0; k < 3; ++k) s += RtD[i * 3 + k] * R[k * 3 + j];
-                W.T[lane * 9 + i * 3 + j] = wq * s;
+                for (int k = 0; k < DIM; ++k) s += RtD[i * DIM + k] * R[k * DIM + j];
+                W.T[lane * DIM * DIM + i * DIM + j] = wq * s;
             }
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < DIM; ++i) {
             double s = 0.0;
-            for (int k = 0; k < 3; ++k) s += R[k * 3 + i] * trac[k];
-            W.ctR[lane * 3 + i] = wq * s;
+            for (int k = 0; k < DIM; ++k) s += R[k * DIM + i] * trac[k];
+            W.ctR[lane * DIM + i] = wq * s;
         }
     }
     __syncthreads();
     if (live && DO_K) {
         double* ke = a.ke + a.ke_ptr[c];
-        if (M <= 32) mfma_ke<2>(W, M, lane, ke);
-        else mfma_ke<3>(W, M, lane, ke);
+        if (M <= 16) mfma_ke<DIM, 1>(W, M, lane, ke);
+        else if (M <= 32) mfma_ke<DIM, 2>(W, M, lane, ke);
+        else mfma_ke<DIM, 3>(W, M, lane, ke);
     }
     if (live && DO_F) {
         double* pe = a.pe + a.pe_ptr[c];
         for (int i = lane; i < M; i += WAVE) {
-            const int an = i / 3, d = i - 3 * an;
+            const int an = i / DIM, d = i - DIM * an;
             double acc = 0.0;
 #pragma unroll 1
-            for (int q = 0; q < nv; ++q) acc += W.w[q] * force_entry<3>(&W.g[q * MF_MM + an * 3], d, &W.sig[q * 6]);
+            for (int q = 0; q < nv; ++q) acc += W.w[q] * force_entry<DIM>(&W.g[q * MM + an * DIM], d, &W.sig[q * S]);
 #pragma unroll 1
-            for (int p = 0; p < nc; ++p) acc += W.cs[p * MAX_NT + an] * W.ctR[p * 3 + d];
+            for (int p = 0; p < nc; ++p) acc += W.cs[p * MAX_NT + an] * W.ctR[p * DIM + d];
             pe[i] = acc;
         }
     }
@@ -1130,15 +1128,10 @@ template <int DIM, bool DO_K, bool DO_F, typename KT>
 static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     if (c.timing_events) (void)hipEventRecord(c.timing_events[0], c.stream);
     if (c.n_cut > 0) {
-        if (DIM == 3) {
-            // 3D: four cut elements per block, K_e on the FP64 matrix cores
-            const int64_t nb = (c.n_cut + pgfk::MF_WAVES - 1) / pgfk::MF_WAVES;
-            hipLaunchKernelGGL((pgfk::k_cut_elem_mfma<DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(nb)),
-                               dim3(pgfk::MF_WAVES * pgfk::WAVE), 0, c.stream, a);
-        } else {
-            hipLaunchKernelGGL((pgfk::k_cut_elem<DIM, DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(c.n_cut)),
-                               dim3(pgfk::cut_threads<DIM>()), 0, c.stream, a);
-        }
+        // four cut elements per block, K_e on the FP64 matrix cores
+        const int64_t nb = (c.n_cut + pgfk::MF_WAVES - 1) / pgfk::MF_WAVES;
+        hipLaunchKernelGGL((pgfk::k_cut_elem_mfma<DIM, DO_K, DO_F>), dim3((unsigned)pgfk::xcd_blocks(nb)),
+                           dim3(pgfk::MF_WAVES * pgfk::WAVE), 0, c.stream, a);
     }
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
