@@ -126,7 +126,7 @@ def main():
     ms = ctx.time_assemble(flags, reps=args.steps)
     value = info.n_elem / (ms * 1e-3)
     # dominant kernel: the row-gather pass (k_gather); cut elements are a
-    # separate, much smaller pass (k_cut_elem)
+    # separate, much smaller pass (k_cut_elem_mfma)
     ms_call, ms_cut, ms_gather = ctx.time_assemble_split(flags, reps=args.steps)
     nnz_free_rows = int(ctx.pattern()[0][info.n_free])
     bytes_gather = gather_bytes(2, nnz_free_rows, info.n_free, info.n_all_nodes, info.n_elem)
@@ -254,7 +254,7 @@ def measured_traffic(workload="c1", kernel="k_gather"):
 
 
 def cut_pass_bytes(prob):
-    """Algorithmic bytes of one cut-element pass (k_cut_elem): per cut element
+    """Algorithmic bytes of one cut-element pass (k_cut_elem_mfma): per cut element
     its integration-point tables (gradients of every slot, J*W, material),
     per cohesive point N, R, dA*W, node map, law, delta_max read and written
     and the flags written, the element's displacements read, and its K_e and
