@@ -90,6 +90,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index; a.image = image;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_rel = inc_rel;
     a.inc_rec = (const int2*)inc_rec; a.el_conn4 = (const int4*)el_conn4; a.Xq = Xq;
+    a.erec_g = erec_g; a.erec_s = erec_s;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
     return a;
@@ -102,7 +103,8 @@ Ctx::~Ctx() {
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
                     inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row, ce_rel, ce_rel_ptr, image,
-                    sens_dgrad, sens_divv, sens_pe, sens_ds, sens_param, inc_rec, el_conn4, Xq};
+                    sens_dgrad, sens_divv, sens_pe, sens_ds, sens_param, inc_rec, el_conn4, Xq,
+                    erec_g, erec_s};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -786,10 +788,13 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         c->ce_off = pgfd::dupload(pat.cut_off.data(), (int64_t)pat.cut_off.size());
         c->ce_off_ptr = pgfd::dupload(pat.cut_off_ptr.data(), (int64_t)pat.cut_off_ptr.size());
         // cut elements: row-relative position of every column (bn, db) in the
-        // rows of slot la, one byte each, [nt][nt * dim] per element
+        // rows of slot la, one byte each, [nt][RS] per element with the row
+        // stride RS = nt * dim rounded up to 4 (rows start 4-byte aligned, so
+        // the gather reads them as 32-bit words)
         {
             std::vector<int64_t> rp(d->n_cut + 1, 0);
-            for (int64_t k = 0; k < d->n_cut; ++k) rp[k + 1] = rp[k] + (int64_t)ce_nt[k] * ce_nt[k] * d->dim;
+            for (int64_t k = 0; k < d->n_cut; ++k)
+                rp[k + 1] = rp[k] + (int64_t)ce_nt[k] * pgfk::cut_rel_stride(ce_nt[k] * d->dim);
             std::vector<uint8_t> rel(std::max<int64_t>(rp[d->n_cut], 1), 0);
             for (int64_t k = 0; k < d->n_cut; ++k) {
                 const int nt = ce_nt[k];
@@ -803,7 +808,7 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
                             const int64_t g = d->dof[(size_t)db * d->n_all_nodes + nd];
                             const int v = g < d->n_free ? of + cf++ : op + cp++;
                             if (v > 255) throw pgf::Error(PGF_ELIMIT, "a matrix row is wider than 256 columns");
-                            rel[rp[k] + (int64_t)la * nt * d->dim + bn * d->dim + db] = (uint8_t)v;
+                            rel[rp[k] + (int64_t)la * pgfk::cut_rel_stride(nt * d->dim) + bn * d->dim + db] = (uint8_t)v;
                         }
                     }
             }
@@ -874,7 +879,7 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             std::vector<int64_t> kp(d->n_cut + 1, 0), pp(d->n_cut + 1, 0);
             for (int64_t k = 0; k < d->n_cut; ++k) {
                 const int64_t m = (int64_t)ce_nt[k] * d->dim;
-                kp[k + 1] = kp[k] + m * m;
+                kp[k + 1] = kp[k] + m * pgfk::cut_ke_stride((int)m);
                 pp[k + 1] = pp[k] + m;
             }
             c->n_ke = kp[d->n_cut];
@@ -1385,6 +1390,30 @@ PGF_API int pgf_set_gather_layout(pgf_ctx* c, int32_t slots, int32_t node_rows) 
         pgfd::build_gather_blocks(*c, slots, node_rows != 0);
         return 0;
     });
+}
+
+PGF_API int pgf_set_element_records(pgf_ctx* c, int32_t on) {
+    if (!c || (on != 0 && on != 1)) return PGF_EARG;
+    return guard(c, [&]() -> int {
+        if (on && c->dim != 3) throw pgf::Error(PGF_EUNSUPPORTED, "element records are built for 3D (TET4) meshes only");
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (on && !c->erec_g) {
+            const int64_t n = std::max<int64_t>(c->n_elem, 1);
+            HIPCHK(hipMalloc((void**)&c->erec_g, sizeof(double) * n * pgfk::EREC_G));
+            HIPCHK(hipMalloc((void**)&c->erec_s, sizeof(double) * n * pgfk::EREC_S));
+        } else if (!on && c->erec_g) {
+            HIPCHK(hipFree(c->erec_g));
+            HIPCHK(hipFree(c->erec_s));
+            c->erec_g = c->erec_s = nullptr;
+        }
+        return 0;
+    });
+}
+
+PGF_API int pgf_get_element_records(pgf_ctx* c, int32_t* on) {
+    if (!c || !on) return PGF_EARG;
+    *on = c->erec_g ? 1 : 0;
+    return 0;
 }
 
 PGF_API int pgf_get_gather_layout(pgf_ctx* c, int32_t* slots, int32_t* node_rows) {

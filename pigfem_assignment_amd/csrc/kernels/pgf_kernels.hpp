@@ -28,6 +28,15 @@ template <int DIM>
 __host__ __device__ constexpr int cut_threads() { return 64; }
 // row-gather blocks: threads (slots x matrix rows) and LDS segment (doubles)
 constexpr int GATHER_THREADS = 256;
+// 3D element records of the gather pass: per uncut tet its 12 gradients and
+// J*W (16 doubles), and its stress (8 doubles)
+constexpr int EREC_G = 16;
+constexpr int EREC_S = 8;
+// cut-element scratch of the gather scheme: K_e rows of M entries stored with
+// an even stride (16-byte aligned rows, read as pairs), and the rows' column
+// bytes with a stride rounded up to 4 (read as 32-bit words)
+__host__ __device__ constexpr int cut_ke_stride(int m) { return (m + 1) & ~1; }
+__host__ __device__ constexpr int cut_rel_stride(int m) { return (m + 3) & ~3; }
 constexpr int SEG_MAX = 4096;
 // MI355X dispatches consecutive workgroups to its 8 XCDs in turn, each XCD
 // with its own L2.  Launching 8*per blocks and mapping block b to logical
@@ -98,6 +107,8 @@ struct AsmArgs {
     const int2* inc_rec;       // [n_inc]
     const int4* el_conn4;      // [n_elem]
     const double* Xq;          // [n_all][2] (2D) or [n_all][4] (3D)
+    double* erec_g;            // [n_elem][EREC_G] or null (3D element records, written by k_tet_records)
+    double* erec_s;            // [n_elem][EREC_S] (their stress, K+f assemblies)
     double* ke;                // cut-element K_e scratch
     const int64_t* ke_ptr;
     double* pe;                // cut-element P_e scratch
