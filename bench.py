@@ -49,30 +49,22 @@ def gather_bytes(dim, nnz_rows, n_rows, n_all, n_elem):
     return 8 * nnz_rows + 8 * n_rows + 2 * 8 * dim * n_all + 17 * n_elem
 
 
-def tune_gather_layout(ctx, flags, reps=5, log=print, records=False):
+def tune_gather_layout(ctx, flags, reps=5, log=print):
     """Time the gather pass for every layout, 1, 2, 4 or 8 threads per unit
-    and a unit being one matrix row or all rows of a node, in 3D also with
-    and without the per-element records (results are bit-identical), and keep
-    the fastest; untimed set-up, like an autotuner."""
+    and a unit being one matrix row or all rows of a node (results are
+    bit-identical), and keep the fastest; untimed set-up, like an autotuner."""
     ms = {}
-    layouts = [(slots, node_rows, rec) for rec in ((0, 1) if records else (0,))
-               for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
+    layouts = [(slots, node_rows) for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
     names = {}
-    for slots, node_rows, rec in layouts:
-        if records:
-            ctx.element_records = rec
+    for slots, node_rows in layouts:
         ctx.gather_layout = (slots, node_rows)
         ctx.assemble(flags)
-        name = "%s%d%s" % ("node" if node_rows else "row", slots, "+rec" if rec else "")
-        names[name] = (slots, node_rows, rec)
+        name = "%s%d" % ("node" if node_rows else "row", slots)
+        names[name] = (slots, node_rows)
         ms[name] = ctx.time_assemble_split(flags, reps=reps)[2]
     best = min(ms, key=ms.get)
-    slots, node_rows, rec = names[best]
-    if records:
-        ctx.element_records = rec
-    ctx.gather_layout = (slots, node_rows)
-    log("# gather pass ms by layout (threads per row / per node%s): %s -> %s" % (
-        "; +rec: per-element records" if records else "",
+    ctx.gather_layout = names[best]
+    log("# gather pass ms by layout (threads per row / per node): %s -> %s" % (
         ", ".join("%s: %.3f" % (k, v) for k, v in ms.items()), best))
     return best, ms
 
@@ -426,7 +418,7 @@ def bench_c2(args):
               info.n_elem, info.n_cut, info.n_free, info.n_cpt, setup_s, args.setup, *info.phase_ms), flush=True)
     ctx = DeviceContext(prob)
     flags = L.PGF_ASM_JAC | L.PGF_ASM_FUNC
-    slots, slot_ms = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True), records=True)
+    slots, slot_ms = tune_gather_layout(ctx, flags, log=lambda m: print(m, flush=True))
     for _ in range(args.warmup):
         ctx.assemble(flags, dt=0.05)
     ms = ctx.time_assemble(flags, reps=args.steps)

@@ -295,14 +295,6 @@ PGF_API int pgf_time_spmv(pgf_ctx* ctx, int32_t precision, int32_t reps, double*
  * node-block partition; the assembled values do not change (bit for bit). */
 PGF_API int pgf_set_gather_layout(pgf_ctx* ctx, int32_t slots, int32_t node_rows);
 PGF_API int pgf_get_gather_layout(pgf_ctx* ctx, int32_t* slots, int32_t* node_rows);
-/* 3D element records: on = 1 evaluates each uncut tet's gradients, J*W and
- * stress once per assembly into per-element records (24 doubles per element,
- * 2.4 GB for 12.6 M tets) that the row-gather pass reads instead of
- * recomputing the geometry for each of the tet's twelve rows; on = 0 frees
- * them.  The assembled values do not change (bit for bit).  3D contexts only
- * (PGF_EUNSUPPORTED for 2D). */
-PGF_API int pgf_set_element_records(pgf_ctx* ctx, int32_t on);
-PGF_API int pgf_get_element_records(pgf_ctx* ctx, int32_t* on);
 
 #ifdef __cplusplus
 }

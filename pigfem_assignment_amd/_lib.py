@@ -100,7 +100,7 @@ EXPORTS = [
     "pgf_assemble", "pgf_get_matrix", "pgf_get_matrix32", "pgf_get_internal_force", "pgf_n_cohesive_points",
     "pgf_isv_commit", "pgf_isv_rollback", "pgf_get_isv", "pgf_set_isv", "pgf_spmv", "pgf_pcg", "pgf_set_guess",
     "pgf_newton_step", "pgf_time_assemble", "pgf_time_assemble_split", "pgf_time_spmv",
-    "pgf_set_gather_layout", "pgf_get_gather_layout", "pgf_set_element_records", "pgf_get_element_records",
+    "pgf_set_gather_layout", "pgf_get_gather_layout",
     "pgfh_problem_create", "pgfh_problem_destroy", "pgfh_last_error", "pgfh_problem_info", "pgfh_mesh_desc",
     "pgfh_get_coords", "pgfh_get_conn", "pgfh_get_elem_mat", "pgfh_get_node_detect", "pgfh_get_dofs",
     "pgfh_get_enrich_edges", "pgfh_get_cut", "pgfh_pattern_info", "pgfh_get_pattern", "pgfh_solve_load_steps",
@@ -140,8 +140,6 @@ def lib():
     L.pgf_time_spmv.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
     L.pgf_set_gather_layout.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     L.pgf_get_gather_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
-    L.pgf_set_element_records.argtypes = [C.c_void_p, C.c_int32]
-    L.pgf_get_element_records.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.pgfh_problem_create.argtypes = [C.POINTER(HostConfig), C.POINTER(C.c_void_p)]
     L.pgf_net_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
     L.pgf_net_destroy.argtypes = [C.c_void_p]

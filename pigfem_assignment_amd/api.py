@@ -360,19 +360,6 @@ class DeviceContext:
         slots, node_rows = layout
         self._chk(self.lib.pgf_set_gather_layout(self.h, int(slots), int(node_rows)))
 
-    @property
-    def element_records(self):
-        """1 when the 3D row-gather pass reads per-element records (each
-        uncut tet's gradients, J*W and stress evaluated once per assembly),
-        0 when it evaluates the geometry per row.  Same values either way."""
-        on = C.c_int32()
-        self._chk(self.lib.pgf_get_element_records(self.h, C.byref(on)))
-        return on.value
-
-    @element_records.setter
-    def element_records(self, on):
-        self._chk(self.lib.pgf_set_element_records(self.h, int(on)))
-
     def time_spmv(self, precision=64, reps=20):
         ms = C.c_double()
         self._chk(self.lib.pgf_time_spmv(self.h, precision, reps, C.byref(ms)))

@@ -848,41 +848,6 @@ __device__ inline void qp_rows(const double* g, double w, const double* sig, int
     }
 }
 
-// Gather scheme, 3D element records: one thread per uncut tet evaluates its
-// gradients and J*W (and, for the internal force, its constant stress) once
-// per assembly, so the gather pass reads one record per incident element
-// instead of the connectivity, four coordinates and twelve displacements, and
-// does not repeat the geometry for each of the tet's twelve rows.  Same
-// functions as the gather's own evaluation: the records hold the same bits.
-template <bool DO_F>
-__global__ __launch_bounds__(256) void k_tet_records(AsmArgs a) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.n_elem || a.cut_index[e] >= 0) return;
-    const int4 cn = a.el_conn4[e];
-    const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
-    double X[4][3];
-    for (int q = 0; q < 4; ++q) {
-        const double4 xv = ((const double4*)a.Xq)[nd[q]];
-        X[q][0] = xv.x;
-        X[q][1] = xv.y;
-        X[q][2] = xv.z;
-    }
-    double g[12], w;
-    qp_geometry<3>(X, a.tet_scale, 0, g, w);
-    double* eg = a.erec_g + e * EREC_G;
-    for (int i = 0; i < 6; ++i) ((double2*)eg)[i] = double2{g[2 * i], g[2 * i + 1]};
-    ((double2*)eg)[6] = double2{w, 0.0};
-    if (DO_F) {
-        double u[12], eps[6], sig[6];
-        for (int q = 0; q < 4; ++q)
-            for (int d = 0; d < 3; ++d) u[q * 3 + d] = a.U[d * a.n_all + nd[q]];
-        strain_of<3>(4, g, u, eps);
-        stress_of<3>(iso_of<3>(a.Dmat + a.el_mat[e] * 36), eps, sig);
-        double* es = a.erec_s + e * EREC_S;
-        for (int i = 0; i < 3; ++i) ((double2*)es)[i] = double2{sig[2 * i], sig[2 * i + 1]};
-    }
-}
-
 // Gather scheme, pass 2.  A block owns a run of consecutive nodes.  Its
 // threads are (unit, slot) pairs: a unit is one matrix row (NR = 1) or all DIM
 // rows of one node (NR = DIM, the element geometry and the nodal data then
@@ -956,84 +921,63 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         double Kc[NR == 1 ? MM : 1];
         uint32_t rc[NR == 1 ? MM / 4 : 1];
         int Mc = 0;
+        int4 rec = {0, 0, 0, 0};
         if (valid) {
-            const int2 rec = a.inc_rec[k];
+            rec = a.inc_rec[k];
             la = rec.y & 0xff;
-            c = (rec.y & 0x100) ? rec.x : -1;
+            c = (rec.y & 0x100) ? 0 : -1;
             if (c < 0) {
                 const int mat = rec.y >> 16;
                 const Iso D = iso_of<DIM>(a.Dmat + mat * 36);
                 for (int h = 0; h < NR; ++h)
                     for (int j = 0; j < 4 * DIM; ++j) Kr[h][j] = 0.0;
-                if (DIM == 3 && a.erec_g) {
-                    // the tet's gradients, J*W and stress, evaluated once for
-                    // all of its rows by k_tet_records
-                    double g[4 * DIM], sig[S];
-                    const double* eg = a.erec_g + (int64_t)rec.x * EREC_G;
-                    for (int i = 0; i < 2 * DIM; ++i) {
-                        const double2 v = ((const double2*)eg)[i];
-                        g[2 * i] = v.x;
-                        g[2 * i + 1] = v.y;
-                    }
-                    const double w = eg[4 * DIM];
-                    if (DO_F) {
-                        const double* es = a.erec_s + (int64_t)rec.x * EREC_S;
-                        for (int i = 0; i < S / 2; ++i) {
-                            const double2 v = ((const double2*)es)[i];
-                            sig[2 * i] = v.x;
-                            sig[2 * i + 1] = v.y;
-                        }
-                    }
-                    qp_rows<DIM, NR, DO_K, DO_F>(g, w, sig, la, da, D, Kr, Pc);
-                } else {
-                    double X[4][DIM];
-                    const int4 cn = a.el_conn4[rec.x];
-                    nd[0] = cn.x; nd[1] = cn.y; nd[2] = cn.z; nd[3] = cn.w;
-                    for (int q = 0; q < 4; ++q) {
-                        if (DIM == 2) {
-                            const double2 xv = ((const double2*)a.Xq)[nd[q]];
-                            X[q][0] = xv.x;
-                            X[q][1] = xv.y;
-                        } else {
-                            const double4 xv = ((const double4*)a.Xq)[nd[q]];
-                            X[q][0] = xv.x;
-                            X[q][1] = xv.y;
-                            X[q][DIM - 1] = xv.z;
-                        }
-                    }
-                    double u[4 * DIM];
-                    if (DO_F)
-                        for (int q = 0; q < 4; ++q)
-                            for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
-                    const bool affine = DIM == 2 && quad_affine((const double(*)[2])X);
-                    if (affine) {
-                        quad_affine_rows<NR>((const double(*)[2])X, la, da, D, (double(*)[8])Kr);
-                        if (DO_F)
-                            for (int h = 0; h < NR; ++h)
-                                for (int j = 0; j < 4 * DIM; ++j) Pc[h] += Kr[h][j] * u[j];
-                    }
-                    // one quadrature point at a time keeps the register footprint small
-#pragma unroll 1
-                    for (int q = 0; q < (affine ? 0 : NQ); ++q) {
-                        double g[4 * DIM], w;
-                        qp_geometry<DIM>(X, a.tet_scale, q, g, w);
-                        double sig[S];
-                        if (DO_F) {
-                            double eps[S];
-                            strain_of<DIM>(4, g, u, eps);
-                            stress_of<DIM>(D, eps, sig);
-                        }
-                        qp_rows<DIM, NR, DO_K, DO_F>(g, w, sig, la, da, D, Kr, Pc);
+                double X[4][DIM];
+                const int4 cn = a.el_conn4[rec.x];
+                nd[0] = cn.x; nd[1] = cn.y; nd[2] = cn.z; nd[3] = cn.w;
+                for (int q = 0; q < 4; ++q) {
+                    if (DIM == 2) {
+                        const double2 xv = ((const double2*)a.Xq)[nd[q]];
+                        X[q][0] = xv.x;
+                        X[q][1] = xv.y;
+                    } else {
+                        const double4 xv = ((const double4*)a.Xq)[nd[q]];
+                        X[q][0] = xv.x;
+                        X[q][1] = xv.y;
+                        X[q][DIM - 1] = xv.z;
                     }
                 }
-            } else {
+                double u[4 * DIM];
                 if (DO_F)
-                    for (int h = 0; h < NR; ++h) Pc[h] = a.pe[a.pe_ptr[c] + la * DIM + da[h]];
+                    for (int q = 0; q < 4; ++q)
+                        for (int d = 0; d < DIM; ++d) u[q * DIM + d] = a.U[d * a.n_all + nd[q]];
+                const bool affine = DIM == 2 && quad_affine((const double(*)[2])X);
+                if (affine) {
+                    quad_affine_rows<NR>((const double(*)[2])X, la, da, D, (double(*)[8])Kr);
+                    if (DO_F)
+                        for (int h = 0; h < NR; ++h)
+                            for (int j = 0; j < 4 * DIM; ++j) Pc[h] += Kr[h][j] * u[j];
+                }
+                // one quadrature point at a time keeps the register footprint small
+#pragma unroll 1
+                for (int q = 0; q < (affine ? 0 : NQ); ++q) {
+                    double g[4 * DIM], w;
+                    qp_geometry<DIM>(X, a.tet_scale, q, g, w);
+                    double sig[S];
+                    if (DO_F) {
+                        double eps[S];
+                        strain_of<DIM>(4, g, u, eps);
+                        stress_of<DIM>(D, eps, sig);
+                    }
+                    qp_rows<DIM, NR, DO_K, DO_F>(g, w, sig, la, da, D, Kr, Pc);
+                }
+            } else {
+                // the record holds the scratch offsets of this slot's rows
+                Mc = (rec.y >> 9) & 0x7f;
+                if (DO_F)
+                    for (int h = 0; h < NR; ++h) Pc[h] = a.pe[rec.w + da[h]];
                 if (DO_K && NR == 1) {
-                    Mc = a.ce_nt[c] * DIM;
-                    const double2* kr =
-                        (const double2*)(a.ke + a.ke_ptr[c] + (int64_t)(la * DIM + da[0]) * cut_ke_stride(Mc));
-                    const uint32_t* rw = (const uint32_t*)(a.ce_rel + a.ce_rel_ptr[c] + (int64_t)la * cut_rel_stride(Mc));
+                    const double2* kr = (const double2*)(a.ke + rec.x + (int64_t)da[0] * cut_ke_stride(Mc));
+                    const uint32_t* rw = (const uint32_t*)(a.ce_rel + rec.z);
 #pragma unroll
                     for (int j = 0; j < MM / 2; ++j)
                         if (2 * j < Mc) {
@@ -1072,13 +1016,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                         for (int j = 0; j < MM; ++j)
                             if (j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
                     } else {
-                        const int M = a.ce_nt[c] * DIM;
-                        const uint8_t* rl = a.ce_rel + a.ce_rel_ptr[c] + (int64_t)la * cut_rel_stride(M);
-                        const double* kb = a.ke + a.ke_ptr[c];
-                        for (int j = 0; j < M; ++j) {
+                        const uint8_t* rl = a.ce_rel + rec.z;
+                        const double* kb = a.ke + rec.x;
+                        for (int j = 0; j < Mc; ++j) {
                             const int rel = rl[j];
                             for (int h = 0; h < NR; ++h)
-                                if (own[h]) add(h, rel, kb[(int64_t)(la * DIM + da[h]) * cut_ke_stride(M) + j]);
+                                if (own[h]) add(h, rel, kb[(int64_t)da[h] * cut_ke_stride(Mc) + j]);
                         }
                     }
                 }
@@ -1241,8 +1184,6 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
                            dim3(pgfk::MF_WAVES * pgfk::WAVE), 0, c.stream, a);
     }
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
-    if (DIM == 3 && c.erec_g && c.n_elem > 0)
-        hipLaunchKernelGGL((pgfk::k_tet_records<DO_F>), dim3((unsigned)((c.n_elem + 255) / 256)), dim3(256), 0, c.stream, a);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
     // regular-node blocks, then enrichment-node blocks (rows filled from the
     // cut elements' K_e only): two launches of the same kernel

@@ -89,8 +89,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.rowptr = rowptr; a.K = K; a.K32 = K32; a.P = P;
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index; a.image = image;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_rel = inc_rel;
-    a.inc_rec = (const int2*)inc_rec; a.el_conn4 = (const int4*)el_conn4; a.Xq = Xq;
-    a.erec_g = erec_g; a.erec_s = erec_s;
+    a.inc_rec = (const int4*)inc_rec; a.el_conn4 = (const int4*)el_conn4; a.Xq = Xq;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
     return a;
@@ -103,8 +102,7 @@ Ctx::~Ctx() {
                     upres_full, r, x, z, p0, p1, q, dinv, guess, res, partial,
                     scal, counter, done, el_conn, cut_index, inc_elem, el_mat, inc_slot, inc_ptr, ke_ptr, pe_ptr,
                     inc_rel, ke, pe, blk_node, blk_seg, blk_rounds, grp_row, ce_rel, ce_rel_ptr, image,
-                    sens_dgrad, sens_divv, sens_pe, sens_ds, sens_param, inc_rec, el_conn4, Xq,
-                    erec_g, erec_s};
+                    sens_dgrad, sens_divv, sens_pe, sens_ds, sens_param, inc_rec, el_conn4, Xq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -832,14 +830,39 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->inc_elem = pgfd::dupload(pat.inc_elem.data(), c->n_inc);
             c->inc_slot = pgfd::dupload(pat.inc_slot.data(), c->n_inc);
             {
-                // packed records of the gather pass (one 8-byte load per
-                // incidence, one 16-byte load per element, one or two per node)
-                std::vector<int32_t> rec((size_t)std::max<int64_t>(c->n_inc, 1) * 2, 0);
+                // packed records of the gather pass: one 16-byte load per
+                // incidence, one 16-byte load per element, one or two per node.
+                // Uncut incidence: (element, slot | material << 16, 0, 0).  Cut
+                // incidence: (first K_e row of the slot in the scratch, slot |
+                // 0x100 | M << 9 | material << 16, first column byte of the
+                // slot's rows, first P_e entry of the slot), so the gather reads
+                // the K_e row without a load of the element's scratch offsets.
+                // The offsets follow the scratch layout set up below (ke_ptr,
+                // ce_rel_ptr, pe_ptr).
+                std::vector<int64_t> kb(d->n_cut + 1, 0), rb(d->n_cut + 1, 0), pb(d->n_cut + 1, 0);
+                for (int64_t k = 0; k < d->n_cut; ++k) {
+                    const int m = ce_nt[k] * d->dim;
+                    kb[k + 1] = kb[k] + (int64_t)m * pgfk::cut_ke_stride(m);
+                    rb[k + 1] = rb[k] + (int64_t)ce_nt[k] * pgfk::cut_rel_stride(m);
+                    pb[k + 1] = pb[k] + m;
+                }
+                if (kb[d->n_cut] > INT32_MAX || rb[d->n_cut] > INT32_MAX)
+                    throw pgf::Error(PGF_ELIMIT, "cut-element scratch beyond 2^31 entries");
+                std::vector<int32_t> rec((size_t)std::max<int64_t>(c->n_inc, 1) * 4, 0);
                 for (int64_t k = 0; k < c->n_inc; ++k) {
                     const int64_t e = pat.inc_elem[k];
                     const int32_t ci = mv.cut_index[e];
-                    rec[2 * k] = ci >= 0 ? ci : (int32_t)e;
-                    rec[2 * k + 1] = (int32_t)pat.inc_slot[k] | (ci >= 0 ? 0x100 : 0) | ((int32_t)d->elem_mat[e] << 16);
+                    const int32_t sl = pat.inc_slot[k], mat = (int32_t)d->elem_mat[e] << 16;
+                    if (ci >= 0) {
+                        const int m = ce_nt[ci] * d->dim;
+                        rec[4 * k] = (int32_t)(kb[ci] + (int64_t)sl * d->dim * pgfk::cut_ke_stride(m));
+                        rec[4 * k + 1] = sl | 0x100 | (m << 9) | mat;
+                        rec[4 * k + 2] = (int32_t)(rb[ci] + (int64_t)sl * pgfk::cut_rel_stride(m));
+                        rec[4 * k + 3] = (int32_t)(pb[ci] + (int64_t)sl * d->dim);
+                    } else {
+                        rec[4 * k] = (int32_t)e;
+                        rec[4 * k + 1] = sl | mat;
+                    }
                 }
                 c->inc_rec = pgfd::dupload(rec.data(), (int64_t)rec.size());
                 c->el_conn4 = pgfd::dupload(d->conn, 4 * std::max<int64_t>(d->n_elem, 1));
@@ -1390,30 +1413,6 @@ PGF_API int pgf_set_gather_layout(pgf_ctx* c, int32_t slots, int32_t node_rows) 
         pgfd::build_gather_blocks(*c, slots, node_rows != 0);
         return 0;
     });
-}
-
-PGF_API int pgf_set_element_records(pgf_ctx* c, int32_t on) {
-    if (!c || (on != 0 && on != 1)) return PGF_EARG;
-    return guard(c, [&]() -> int {
-        if (on && c->dim != 3) throw pgf::Error(PGF_EUNSUPPORTED, "element records are built for 3D (TET4) meshes only");
-        HIPCHK(hipStreamSynchronize(c->stream));
-        if (on && !c->erec_g) {
-            const int64_t n = std::max<int64_t>(c->n_elem, 1);
-            HIPCHK(hipMalloc((void**)&c->erec_g, sizeof(double) * n * pgfk::EREC_G));
-            HIPCHK(hipMalloc((void**)&c->erec_s, sizeof(double) * n * pgfk::EREC_S));
-        } else if (!on && c->erec_g) {
-            HIPCHK(hipFree(c->erec_g));
-            HIPCHK(hipFree(c->erec_s));
-            c->erec_g = c->erec_s = nullptr;
-        }
-        return 0;
-    });
-}
-
-PGF_API int pgf_get_element_records(pgf_ctx* c, int32_t* on) {
-    if (!c || !on) return PGF_EARG;
-    *on = c->erec_g ? 1 : 0;
-    return 0;
 }
 
 PGF_API int pgf_get_gather_layout(pgf_ctx* c, int32_t* slots, int32_t* node_rows) {

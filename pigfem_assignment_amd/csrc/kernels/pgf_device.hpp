@@ -45,7 +45,6 @@ struct Ctx {
     int32_t *el_conn = nullptr, *cut_index = nullptr, *inc_elem = nullptr;
     int32_t *inc_rec = nullptr, *el_conn4 = nullptr;  // gather records (see AsmArgs)
     double* Xq = nullptr;
-    double *erec_g = nullptr, *erec_s = nullptr;  // 3D element records (pgf_set_element_records)
     uint8_t *el_mat = nullptr, *inc_slot = nullptr;
     int64_t *inc_ptr = nullptr, *ke_ptr = nullptr, *pe_ptr = nullptr;
     uint8_t* inc_rel = nullptr;  // [n_inc][4 * dim] row-relative column positions (uncut elements)

@@ -28,10 +28,6 @@ template <int DIM>
 __host__ __device__ constexpr int cut_threads() { return 64; }
 // row-gather blocks: threads (slots x matrix rows) and LDS segment (doubles)
 constexpr int GATHER_THREADS = 256;
-// 3D element records of the gather pass: per uncut tet its 12 gradients and
-// J*W (16 doubles), and its stress (8 doubles)
-constexpr int EREC_G = 16;
-constexpr int EREC_S = 8;
 // cut-element scratch of the gather scheme: K_e rows of M entries stored with
 // an even stride (16-byte aligned rows, read as pairs), and the rows' column
 // bytes with a stride rounded up to 4 (read as 32-bit words)
@@ -101,14 +97,14 @@ struct AsmArgs {
     const int32_t* inc_elem;
     const uint8_t* inc_slot;
     const uint8_t* inc_rel;    // [n_inc][4 * dim] row-relative column positions (uncut elements)
-    // gather records: per incidence (element or cut-element index, slot |
-    // cut flag << 8 | material << 16); per element its 4 nodes; per node its
+    // gather records: per incidence (element, slot | material << 16, 0, 0) or,
+    // for a cut element, (first K_e row of the slot in the scratch, slot |
+    // 0x100 | M << 9 | material << 16, first column byte of the slot's rows,
+    // first P_e entry of the slot); per element its 4 nodes; per node its
     // coordinates padded to 2 or 4 doubles (one or two 16-byte loads)
-    const int2* inc_rec;       // [n_inc]
+    const int4* inc_rec;       // [n_inc]
     const int4* el_conn4;      // [n_elem]
     const double* Xq;          // [n_all][2] (2D) or [n_all][4] (3D)
-    double* erec_g;            // [n_elem][EREC_G] or null (3D element records, written by k_tet_records)
-    double* erec_s;            // [n_elem][EREC_S] (their stress, K+f assemblies)
     double* ke;                // cut-element K_e scratch
     const int64_t* ke_ptr;
     double* pe;                // cut-element P_e scratch

@@ -141,10 +141,9 @@ def test_3d_cohesive_stiffness(law, scheme):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_gather_layouts_bit_identical(dim):
     """Every layout of the gather pass (1, 2, 4, 8 threads per unit; a unit one
-    matrix row or all rows of a node; in 3D with and without the per-element
-    records) sums each entry's element contributions in element order: K, P
-    and the cohesive state are bit-identical across layouts, and match the
-    oracle."""
+    matrix row or all rows of a node) sums each entry's element contributions
+    in element order: K, P and the cohesive state are bit-identical across
+    layouts, and match the oracle."""
     p, o = _c0(60) if dim == 2 else _c2small(configs.BILINEAR_C2)
     ctx = DeviceContext(p)
     rng = np.random.RandomState(5)
@@ -154,13 +153,9 @@ def test_gather_layouts_bit_identical(dim):
     vals, Po, _ = o.assemble(dt=0.05)
     rp = ctx.pattern()[0]
     ref = None
-    layouts = [(slots, node_rows, rec) for rec in ((0, 1) if dim == 3 else (0,))
-               for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
-    for slots, node_rows, rec in layouts:
+    layouts = [(slots, node_rows) for node_rows in (0, 1) for slots in (1, 2, 4, 8)]
+    for slots, node_rows in layouts:
         if True:
-            if dim == 3:
-                ctx.element_records = rec
-                assert ctx.element_records == rec
             ctx.gather_layout = (slots, node_rows)
             assert ctx.gather_layout == (slots, node_rows)
             for flags in (L.PGF_ASM_JAC | L.PGF_ASM_FUNC | L.PGF_ASM_ALL_ROWS, L.PGF_ASM_JAC, L.PGF_ASM_FUNC):
@@ -171,7 +166,7 @@ def test_gather_layouts_bit_identical(dim):
             # rows are summed outside the LDS segments)
             ctx.assemble(L.PGF_ASM_JAC | L.PGF_ASM_FP32 | L.PGF_ASM_ALL_ROWS, dt=0.05)
             assert np.array_equal(ctx.matrix32(), got[0].astype(np.float32))
-            print("dim %d slots %d node_rows %d records %d" % (dim, slots, node_rows, rec), flush=True)
+            print("dim %d slots %d node_rows %d" % (dim, slots, node_rows), flush=True)
             if ref is None:
                 ref = got
                 assert k_entries_ok(got[0], o, rp, "layout reference", dt=0.05, vals=vals)
@@ -188,12 +183,6 @@ def test_gather_layouts_bit_identical(dim):
         ctx.gather_layout = (3, 0)
     with pytest.raises(RuntimeError):
         ctx.gather_layout = (1, 2)
-    if dim == 2:
-        with pytest.raises(RuntimeError, match="3D"):
-            ctx.element_records = 1
-    else:
-        ctx.element_records = 0
-        assert ctx.element_records == 0
 
 
 @pytest.mark.parametrize("precision", [64, 32])
