@@ -858,10 +858,10 @@ __device__ inline void qp_rows(const double* g, double w, const double* sig, int
 // form one contiguous CSR segment [s0, s1) it is accumulated in LDS and
 // written back with coalesced stores; otherwise (s0 == s1), and for
 // prescribed rows (all_rows only), the rows are accumulated in place in HBM.
-// The slots add their element's contribution in slot order between barriers,
-// so every entry sums its incident elements in element order, the reference's
-// summation order, and the result does not depend on scheduling (with one
-// slot a thread owns its rows and needs no barriers between elements).
+// The slots add their element's contribution in slot order, so every entry
+// sums its incident elements in the node's incidence order (its uncut
+// elements in element order, then its cut elements in element order) and the
+// result does not depend on the layout or on scheduling.
 template <int DIM, int SL, int NR, bool DO_K, bool DO_F, typename KT>
 __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     HIP_DYNAMIC_SHARED(double, s_seg)
@@ -873,6 +873,14 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
     const int64_t s0 = a.blk_seg[2 * blk], s1 = a.blk_seg[2 * blk + 1];
     const int rounds = a.blk_rounds[blk];
     const bool use_lds = DO_K && s1 > s0;
+    // The slots of a unit are consecutive lanes of one wavefront and a unit's
+    // rows are touched by its own lanes only.  When every row the block sums
+    // lives in LDS, the slot order needs no block barrier: the wavefront
+    // issues slot 0's LDS operations before slot 1's and LDS executes one
+    // wavefront's instructions in order, so each wavefront runs its rounds
+    // without waiting for the block's slowest.  Rows summed in HBM (prescribed
+    // rows, blocks without a contiguous segment) keep the block barrier.
+    const bool block_sync = DO_K && (!use_lds || a.all_rows);
     const int t = threadIdx.x;
     const int lu = t / SL, slot = t % SL;  // local unit, slot
     const int64_t n = NR == 1 ? n0 + lu / DIM : n0 + lu;
@@ -1026,13 +1034,17 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                     }
                 }
             }
-            if (SL > 1) __syncthreads();
+            if (SL > 1) {
+                if (block_sync) __syncthreads();
+                else __builtin_amdgcn_wave_barrier();  // keeps the slots' code in slot order
+            }
         }
     }
     if (DO_F && slot == 0)
         for (int h = 0; h < NR; ++h)
             if (own[h]) a.P[r[h]] = s_P[lu * NR + h];
-    // (with several slots the last round ended on a barrier: every add is done)
+    // (rows summed in HBM: with several slots the last round ended on a block
+    // barrier, every add is done)
     if (DO_K && sizeof(KT) == 4 && slot == 0)
         for (int h = 0; h < NR; ++h)
             if (own[h] && !in_lds[h])

@@ -825,6 +825,32 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->el_conn = pgfd::dupload(ec.data(), (int64_t)ec.size());
             c->el_mat = pgfd::dupload(em.data(), (int64_t)em.size());
             c->cut_index = pgfd::dupload(mv.cut_index.data(), d->n_elem);
+            // each node's incidences: its uncut elements first, then its cut
+            // elements, each in element order (the order in which a row's
+            // entries are summed).  The gather's rounds then meet a row's cut
+            // elements (whose K_e rows come from HBM) together at the end of
+            // the row, not scattered between the uncut elements, so fewer
+            // rounds mix the two kinds of work in one wavefront.
+            {
+                const size_t ni = pat.inc_elem.size();
+                std::vector<int32_t> ie(ni);
+                std::vector<uint8_t> is(ni);
+                std::vector<uint64_t> io(ni);
+                for (size_t n = 0; n + 1 < pat.inc_ptr.size(); ++n) {
+                    int64_t w = pat.inc_ptr[n];
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int64_t k = pat.inc_ptr[n]; k < pat.inc_ptr[n + 1]; ++k)
+                            if ((mv.cut_index[pat.inc_elem[k]] >= 0) == (pass == 1)) {
+                                ie[w] = pat.inc_elem[k];
+                                is[w] = pat.inc_slot[k];
+                                io[w] = pat.inc_off[k];
+                                ++w;
+                            }
+                }
+                pat.inc_elem.swap(ie);
+                pat.inc_slot.swap(is);
+                pat.inc_off.swap(io);
+            }
             c->n_inc = (int64_t)pat.inc_elem.size();
             c->inc_ptr = pgfd::dupload(pat.inc_ptr.data(), (int64_t)pat.inc_ptr.size());
             c->inc_elem = pgfd::dupload(pat.inc_elem.data(), c->n_inc);
