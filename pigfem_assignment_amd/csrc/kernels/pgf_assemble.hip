@@ -918,16 +918,20 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         const int64_t k = kbeg + (int64_t)rr * SL + slot;
         const bool valid = any && k < kend;
         // ---- compute: this slot's element contribution to the unit's rows ----
-        double Kr[NR][4 * DIM];
         int nd[4];
         double Pc[NR];
         for (int h = 0; h < NR; ++h) Pc[h] = 0.0;
         int c = -1, la = 0;
-        // one row of a cut element's K_e and its column bytes, loaded together
-        // (one memory latency per cut incidence instead of one per entry)
+        // One unit row (NR == 1): the row's entries from this incidence and
+        // their column bytes, in one buffer for both element kinds (an uncut
+        // element's 4 * DIM entries, or a cut element's K_e row loaded in one
+        // batch: one memory latency per cut incidence instead of one per
+        // entry), so both kinds share one add loop and one set of registers.
         constexpr int MM = MAX_NT * DIM;
         double Kc[NR == 1 ? MM : 1];
         uint32_t rc[NR == 1 ? MM / 4 : 1];
+        double Kn[NR == 1 ? 1 : NR][4 * DIM];
+        double (*Kr)[4 * DIM] = NR == 1 ? reinterpret_cast<double (*)[4 * DIM]>(Kc) : Kn;
         int Mc = 0;
         int4 rec = {0, 0, 0, 0};
         if (valid) {
@@ -939,6 +943,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                 const Iso D = iso_of<DIM>(a.Dmat + mat * 36);
                 for (int h = 0; h < NR; ++h)
                     for (int j = 0; j < 4 * DIM; ++j) Kr[h][j] = 0.0;
+                if (DO_K && NR == 1) {
+                    // row-relative column positions, one byte per column
+                    const uint32_t* rw = (const uint32_t*)(a.inc_rel + k * (4 * DIM));
+                    for (int i = 0; i < DIM; ++i) rc[i] = rw[i];
+                    Mc = 4 * DIM;
+                }
                 double X[4][DIM];
                 const int4 cn = a.el_conn4[rec.x];
                 nd[0] = cn.x; nd[1] = cn.y; nd[2] = cn.z; nd[3] = cn.w;
@@ -1009,7 +1019,11 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                         if (in_lds[h]) lds_add(&s_seg[base[h] + rel - s0], v);
                         else a.K[base[h] + rel] += v;
                     };
-                    if (c < 0) {
+                    if (NR == 1) {
+#pragma unroll
+                        for (int j = 0; j < MM; ++j)
+                            if (j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
+                    } else if (c < 0) {
                         // row-relative column positions, one byte per column
                         uint32_t wv[DIM];
                         const uint32_t* rw = (const uint32_t*)(a.inc_rel + k * (4 * DIM));
@@ -1019,10 +1033,6 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                             for (int h = 0; h < NR; ++h)
                                 if (own[h]) add(h, rel, Kr[h][j]);
                         }
-                    } else if (NR == 1) {
-#pragma unroll
-                        for (int j = 0; j < MM; ++j)
-                            if (j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
                     } else {
                         const uint8_t* rl = a.ce_rel + rec.z;
                         const double* kb = a.ke + rec.x;
