@@ -927,9 +927,13 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         // element's 4 * DIM entries, or a cut element's K_e row loaded in one
         // batch: one memory latency per cut incidence instead of one per
         // entry), so both kinds share one add loop and one set of registers.
+        // In 3D the buffer holds the first CH = 20 entries; the rest of a cut
+        // row (up to 16) is loaded in the add phase, which keeps the gather at
+        // three waves per SIMD.
         constexpr int MM = MAX_NT * DIM;
-        double Kc[NR == 1 ? MM : 1];
-        uint32_t rc[NR == 1 ? MM / 4 : 1];
+        constexpr int CH = DIM == 3 ? 20 : MM;
+        double Kc[NR == 1 ? CH : 1];
+        uint32_t rc[NR == 1 ? CH / 4 : 1];
         double Kn[NR == 1 ? 1 : NR][4 * DIM];
         double (*Kr)[4 * DIM] = NR == 1 ? reinterpret_cast<double (*)[4 * DIM]>(Kc) : Kn;
         int Mc = 0;
@@ -997,14 +1001,14 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                     const double2* kr = (const double2*)(a.ke + rec.x + (int64_t)da[0] * cut_ke_stride(Mc));
                     const uint32_t* rw = (const uint32_t*)(a.ce_rel + rec.z);
 #pragma unroll
-                    for (int j = 0; j < MM / 2; ++j)
+                    for (int j = 0; j < CH / 2; ++j)
                         if (2 * j < Mc) {
                             const double2 v = kr[j];
                             Kc[2 * j] = v.x;
                             Kc[2 * j + 1] = v.y;
                         }
 #pragma unroll
-                    for (int j = 0; j < MM / 4; ++j)
+                    for (int j = 0; j < CH / 4; ++j)
                         if (4 * j < Mc) rc[j] = rw[j];
                 }
             }
@@ -1021,8 +1025,27 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                     };
                     if (NR == 1) {
 #pragma unroll
-                        for (int j = 0; j < MM; ++j)
+                        for (int j = 0; j < CH; ++j)
                             if (j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
+                        if (CH < MM && Mc > CH) {
+                            // the rest of a cut element's row
+                            const double2* kr =
+                                (const double2*)(a.ke + rec.x + (int64_t)da[0] * cut_ke_stride(Mc)) + CH / 2;
+                            const uint32_t* rw = (const uint32_t*)(a.ce_rel + rec.z) + CH / 4;
+#pragma unroll
+                            for (int j = 0; j < (MM - CH) / 2; ++j)
+                                if (CH + 2 * j < Mc) {
+                                    const double2 v = kr[j];
+                                    Kc[2 * j] = v.x;
+                                    Kc[2 * j + 1] = v.y;
+                                }
+#pragma unroll
+                            for (int j = 0; j < (MM - CH) / 4; ++j)
+                                if (CH + 4 * j < Mc) rc[j] = rw[j];
+#pragma unroll
+                            for (int j = 0; j < MM - CH; ++j)
+                                if (CH + j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
+                        }
                     } else if (c < 0) {
                         // row-relative column positions, one byte per column
                         uint32_t wv[DIM];
