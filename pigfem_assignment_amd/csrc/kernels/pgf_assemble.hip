@@ -927,11 +927,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         // element's 4 * DIM entries, or a cut element's K_e row loaded in one
         // batch: one memory latency per cut incidence instead of one per
         // entry), so both kinds share one add loop and one set of registers.
-        // In 3D the buffer holds the first CH = 20 entries; the rest of a cut
-        // row (up to 16) is loaded in the add phase, which keeps the gather at
-        // three waves per SIMD.
+        // The buffer holds the first CH entries (20 in 3D, 8 in 2D: an uncut
+        // element's row fits); the rest of a longer cut row (up to 16, or 16)
+        // is loaded in the add phase, which keeps the gather at four waves
+        // per SIMD.
         constexpr int MM = MAX_NT * DIM;
-        constexpr int CH = DIM == 3 ? 20 : MM;
+        constexpr int CH = DIM == 3 ? 20 : 8;
         double Kc[NR == 1 ? CH : 1];
         uint32_t rc[NR == 1 ? CH / 4 : 1];
         double Kn[NR == 1 ? 1 : NR][4 * DIM];
