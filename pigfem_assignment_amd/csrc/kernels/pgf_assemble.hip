@@ -928,11 +928,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         // batch: one memory latency per cut incidence instead of one per
         // entry), so both kinds share one add loop and one set of registers.
         // The buffer holds the first CH entries (20 in 3D, 8 in 2D: an uncut
-        // element's row fits); the rest of a longer cut row (up to 16, or 16)
-        // is loaded in the add phase, which keeps the gather at four waves
-        // per SIMD.
+        // element's row fits); the rest of a longer cut row is loaded in the
+        // add phase, CH entries at a time, which keeps the gather at four
+        // waves per SIMD.
         constexpr int MM = MAX_NT * DIM;
         constexpr int CH = DIM == 3 ? 20 : 8;
+        static_assert(CH % 4 == 0 && CH >= 4 * DIM && CH <= MM, "row buffer: whole words, an uncut row fits");
         double Kc[NR == 1 ? CH : 1];
         uint32_t rc[NR == 1 ? CH / 4 : 1];
         double Kn[NR == 1 ? 1 : NR][4 * DIM];
@@ -1028,24 +1029,29 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
 #pragma unroll
                         for (int j = 0; j < CH; ++j)
                             if (j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
-                        if (CH < MM && Mc > CH) {
-                            // the rest of a cut element's row
-                            const double2* kr =
-                                (const double2*)(a.ke + rec.x + (int64_t)da[0] * cut_ke_stride(Mc)) + CH / 2;
-                            const uint32_t* rw = (const uint32_t*)(a.ce_rel + rec.z) + CH / 4;
+                        // the rest of a cut element's row, CH entries at a time
+                        // (every index stays inside the CH-entry buffer)
+                        constexpr int NCH = (MM - 1) / CH;  // further chunks at most
 #pragma unroll
-                            for (int j = 0; j < (MM - CH) / 2; ++j)
-                                if (CH + 2 * j < Mc) {
+                        for (int ch = 1; ch <= NCH; ++ch) {
+                            const int j0 = ch * CH;
+                            if (j0 >= Mc) break;
+                            const double2* kr =
+                                (const double2*)(a.ke + rec.x + (int64_t)da[0] * cut_ke_stride(Mc) + j0);
+                            const uint32_t* rw = (const uint32_t*)(a.ce_rel + rec.z + j0);
+#pragma unroll
+                            for (int j = 0; j < CH / 2; ++j)
+                                if (j0 + 2 * j < Mc) {
                                     const double2 v = kr[j];
                                     Kc[2 * j] = v.x;
                                     Kc[2 * j + 1] = v.y;
                                 }
 #pragma unroll
-                            for (int j = 0; j < (MM - CH) / 4; ++j)
-                                if (CH + 4 * j < Mc) rc[j] = rw[j];
+                            for (int j = 0; j < CH / 4; ++j)
+                                if (j0 + 4 * j < Mc) rc[j] = rw[j];
 #pragma unroll
-                            for (int j = 0; j < MM - CH; ++j)
-                                if (CH + j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
+                            for (int j = 0; j < CH; ++j)
+                                if (j0 + j < Mc && own[0]) add(0, (int)((rc[j / 4] >> (8 * (j % 4))) & 0xffu), Kc[j]);
                         }
                     } else if (c < 0) {
                         // row-relative column positions, one byte per column
