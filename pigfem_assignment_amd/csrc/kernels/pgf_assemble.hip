@@ -727,9 +727,30 @@ __global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a)
     if (lane < nt) W.node[lane] = a.ce_nodes[c * 16 + lane];
     __syncthreads();
     for (int i = lane; i < M; i += WAVE) W.u[i] = a.U[(i % DIM) * a.n_all + W.node[i / DIM]];
-    for (int k = lane; k < nv * M; k += WAVE) {
-        const int q = k / M;
-        W.g[q * MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
+    if (a.grad_contig) {
+        // one run of nv * M gradients: every load issued before the first is
+        // used (one memory latency, not one per pass of the wave)
+        constexpr int GL = (MfWave<DIM>::NV * MM + WAVE - 1) / WAVE;
+        const double* gs = a.vpt_grad + (live ? a.vpt_grad_off[v0] : 0);
+        double gv[GL];
+#pragma unroll
+        for (int i = 0; i < GL; ++i) {
+            const int k = lane + i * WAVE;
+            if (k < nv * M) gv[i] = gs[k];
+        }
+#pragma unroll
+        for (int i = 0; i < GL; ++i) {
+            const int k = lane + i * WAVE;
+            if (k < nv * M) {
+                const int q = k / M;
+                W.g[q * MM + (k - q * M)] = gv[i];
+            }
+        }
+    } else {
+        for (int k = lane; k < nv * M; k += WAVE) {
+            const int q = k / M;
+            W.g[q * MM + (k - q * M)] = a.vpt_grad[a.vpt_grad_off[v0 + q] + (k - q * M)];
+        }
     }
     if (lane < nv) {
         const double w = a.vpt_w[v0 + lane];

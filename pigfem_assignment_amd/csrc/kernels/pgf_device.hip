@@ -90,6 +90,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index; a.image = image;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_rel = inc_rel;
     a.inc_rec = (const int4*)inc_rec; a.el_conn4 = (const int4*)el_conn4; a.Xq = Xq;
+    a.grad_contig = grad_contig ? 1 : 0;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
     return a;
@@ -725,8 +726,13 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
         const int64_t n_vpt = d->n_cut ? d->vpt_ptr[d->n_cut] : 0;
         const int64_t n_cpt = d->n_cut ? d->cpt_ptr[d->n_cut] : 0;
         int64_t n_grad = 0;
+        // the cut kernel stages an element's gradient blocks as one run when
+        // they are consecutive (as the set-up writes them)
+        c->grad_contig = true;
         for (int64_t k = 0; k < d->n_cut; ++k)
             for (int64_t pt = d->vpt_ptr[k]; pt < d->vpt_ptr[k + 1]; ++pt) {
+                const int64_t m = (int64_t)ce_nt[k] * d->dim;
+                if (d->vpt_grad_off[pt] != d->vpt_grad_off[d->vpt_ptr[k]] + (pt - d->vpt_ptr[k]) * m) c->grad_contig = false;
                 n_grad = std::max<int64_t>(n_grad, d->vpt_grad_off[pt] + (int64_t)ce_nt[k] * d->dim);
                 if (d->vpt_mat[pt] < 0 || d->vpt_mat[pt] >= d->n_mat) throw pgf::Error(PGF_EARG, "point material out of range");
             }

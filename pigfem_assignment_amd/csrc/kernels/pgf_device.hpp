@@ -58,6 +58,7 @@ struct Ctx {
     int64_t n_gblk_reg = 0;         // gather blocks of the regular nodes (the rest: enrichment nodes)
     int gslots = 1;                 // gather pass: threads per row (or per node with gnode)
     bool gnode = false;             // gather pass: one thread serves all rows of a node
+    bool grad_contig = false;       // every cut element's gradient blocks are consecutive
     int64_t n_vpt = 0, n_vgrad = 0;  // volumetric points of the cut elements and their gradient entries
     std::vector<int32_t> h_cut_elem; // [n_cut] parent element of each cut element
     // shape-sensitivity tables and scratch (pgf_sens.hip)

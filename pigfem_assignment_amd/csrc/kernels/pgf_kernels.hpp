@@ -120,6 +120,7 @@ struct AsmArgs {
     float* K32;
     double* P;            // [n_rows]
     int all_rows;
+    int grad_contig;  // 1: a cut element's point gradients are one run starting at vpt_grad_off[first point]
 };
 
 // ---- Ortiz-Pandolfi / bilinear traction-separation law ---------------------
