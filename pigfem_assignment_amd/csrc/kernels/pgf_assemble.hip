@@ -1258,9 +1258,12 @@ static void launch_gather(const Ctx& c, const pgfk::AsmArgs& a) {
     }
     if (c.timing_events) (void)hipEventRecord(c.timing_events[1], c.stream);
     const size_t lds = DO_K ? sizeof(double) * (size_t)c.seg_max : 0;
-    // regular-node blocks, then enrichment-node blocks (rows filled from the
-    // cut elements' K_e only): two launches of the same kernel
-    const int64_t ranges[2][2] = {{0, c.n_gblk_reg}, {c.n_gblk_reg, c.n_gblk}};
+    // 3D: regular-node blocks, then enrichment-node blocks (rows filled from
+    // the cut elements' K_e only), two launches of the same kernel (128^3 x 6
+    // tets with 200 spheres: 6.99 ms against 7.43 ms in one launch); 2D: one
+    // launch over all blocks (1000 x 1000 quads: 0.220 ms against 0.236 ms)
+    const int64_t split = DIM == 3 ? c.n_gblk_reg : c.n_gblk;
+    const int64_t ranges[2][2] = {{0, split}, {split, c.n_gblk}};
     for (const auto& rg : ranges) {
         if (rg[1] <= rg[0]) continue;
         pgfk::AsmArgs ar = a;
