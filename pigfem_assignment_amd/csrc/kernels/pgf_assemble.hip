@@ -701,7 +701,8 @@ __device__ inline void mfma_ke(const MfWave<DIM>& W, int M, int lane, double* ke
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * ti + 4 * r + k4, col = 16 * tj + r16;
-                if (row < M && col < M) ke[row * cut_ke_stride(M) + col] = acc[ti][tj][r];
+                if (row < M && col < M)
+                    ke[(row / DIM) * cut_slot_doubles(M, DIM) + (row % DIM) * cut_ke_stride(M) + col] = acc[ti][tj][r];
             }
 }
 
@@ -818,7 +819,8 @@ __global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a)
         else mfma_ke<DIM, 3>(W, M, lane, ke);
     }
     if (live && DO_F) {
-        double* pe = a.pe + a.pe_ptr[c];
+        // P_e entries of slot an follow its K_e rows and column bytes
+        double* pe = a.ke + a.ke_ptr[c];
         for (int i = lane; i < M; i += WAVE) {
             const int an = i / DIM, d = i - DIM * an;
             double acc = 0.0;
@@ -826,7 +828,7 @@ __global__ __launch_bounds__(MF_WAVES * WAVE, 4) void k_cut_elem_mfma(AsmArgs a)
             for (int q = 0; q < nv; ++q) acc += W.w[q] * force_entry<DIM>(&W.g[q * MM + an * DIM], d, &W.sig[q * S]);
 #pragma unroll 1
             for (int p = 0; p < nc; ++p) acc += W.cs[p * MAX_NT + an] * W.ctR[p * DIM + d];
-            pe[i] = acc;
+            pe[an * cut_slot_doubles(M, DIM) + DIM * cut_ke_stride(M) + cut_rel_doubles(M) + d] = acc;
         }
     }
 }
@@ -960,7 +962,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
         double Kn[NR == 1 ? 1 : NR][4 * DIM];
         double (*Kr)[4 * DIM] = NR == 1 ? reinterpret_cast<double (*)[4 * DIM]>(Kc) : Kn;
         int Mc = 0;
-        int4 rec = {0, 0, 0, 0};
+        int2 rec = {0, 0};
         if (valid) {
             rec = a.inc_rec[k];
             la = rec.y & 0xff;
@@ -1019,10 +1021,11 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                 // the record holds the scratch offsets of this slot's rows
                 Mc = (rec.y >> 9) & 0x7f;
                 if (DO_F)
-                    for (int h = 0; h < NR; ++h) Pc[h] = a.pe[rec.w + da[h]];
+                    for (int h = 0; h < NR; ++h)
+                        Pc[h] = a.ke[rec.x + DIM * cut_ke_stride(Mc) + cut_rel_doubles(Mc) + da[h]];
                 if (DO_K && NR == 1) {
                     const double2* kr = (const double2*)(a.ke + rec.x + (int64_t)da[0] * cut_ke_stride(Mc));
-                    const uint32_t* rw = (const uint32_t*)(a.ce_rel + rec.z);
+                    const uint32_t* rw = (const uint32_t*)(a.ke + rec.x + DIM * cut_ke_stride(Mc));
 #pragma unroll
                     for (int j = 0; j < CH / 2; ++j)
                         if (2 * j < Mc) {
@@ -1059,7 +1062,8 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                             if (j0 >= Mc) break;
                             const double2* kr =
                                 (const double2*)(a.ke + rec.x + (int64_t)da[0] * cut_ke_stride(Mc) + j0);
-                            const uint32_t* rw = (const uint32_t*)(a.ce_rel + rec.z + j0);
+                            const uint32_t* rw =
+                                (const uint32_t*)((const uint8_t*)(a.ke + rec.x + DIM * cut_ke_stride(Mc)) + j0);
 #pragma unroll
                             for (int j = 0; j < CH / 2; ++j)
                                 if (j0 + 2 * j < Mc) {
@@ -1085,7 +1089,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather(AsmArgs a) {
                                 if (own[h]) add(h, rel, Kr[h][j]);
                         }
                     } else {
-                        const uint8_t* rl = a.ce_rel + rec.z;
+                        const uint8_t* rl = (const uint8_t*)(a.ke + rec.x + DIM * cut_ke_stride(Mc));
                         const double* kb = a.ke + rec.x;
                         for (int j = 0; j < Mc; ++j) {
                             const int rel = rl[j];

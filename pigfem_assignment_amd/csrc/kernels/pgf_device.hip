@@ -89,7 +89,7 @@ pgfk::AsmArgs Ctx::asm_args() const {
     a.rowptr = rowptr; a.K = K; a.K32 = K32; a.P = P;
     a.n_elem = n_elem; a.el_conn = el_conn; a.el_mat = el_mat; a.cut_index = cut_index; a.image = image;
     a.inc_ptr = inc_ptr; a.inc_elem = inc_elem; a.inc_slot = inc_slot; a.inc_rel = inc_rel;
-    a.inc_rec = (const int4*)inc_rec; a.el_conn4 = (const int4*)el_conn4; a.Xq = Xq;
+    a.inc_rec = (const int2*)inc_rec; a.el_conn4 = (const int4*)el_conn4; a.Xq = Xq;
     a.grad_contig = grad_contig ? 1 : 0;
     a.ke = ke; a.ke_ptr = ke_ptr; a.pe = pe; a.pe_ptr = pe_ptr;
     a.n_gblk = n_gblk; a.blk_node = blk_node; a.blk_seg = blk_seg; a.blk_rounds = blk_rounds; a.n_cut = n_cut;
@@ -268,6 +268,23 @@ constexpr int CG_PART = CG_SPMV_BLOCKS + 2 * CG_UPD_BLOCKS;  // partial slots of
 // entry is loaded once for all of them (the column index stream shrinks to a
 // third in 3D).  Each row still sums its entries lane-strided and then across
 // the G lanes, the same order as one row per G lanes.
+// Places each cut element's column bytes (compact rows of cut_rel_stride
+// bytes) into its slots' blocks of the gather scratch, once at set-up: the
+// cut kernel writes only the K_e rows and P_e entries of a block.
+__global__ __launch_bounds__(256) void k_place_rel(int64_t n_cut, int dim, const int32_t* __restrict__ ce_nt,
+                                                   const uint8_t* __restrict__ rel, const int64_t* __restrict__ rel_ptr,
+                                                   const int64_t* __restrict__ ke_ptr, double* __restrict__ ke) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n_cut) return;
+    const int nt = ce_nt[k], m = nt * dim, rs = pgfk::cut_rel_stride(m);
+    const int sd = pgfk::cut_slot_doubles(m, dim), ks = pgfk::cut_ke_stride(m);
+    for (int la = 0; la < nt; ++la) {
+        const uint32_t* src = (const uint32_t*)(rel + rel_ptr[k] + (int64_t)la * rs);
+        uint32_t* dst = (uint32_t*)(ke + ke_ptr[k] + (int64_t)la * sd + (int64_t)dim * ks);
+        for (int w = 0; w < rs / 4; ++w) dst[w] = src[w];
+    }
+}
+
 template <typename TA, typename T, int G>
 __global__ __launch_bounds__(256) void k_spmv(int64_t ng, const int32_t* __restrict__ grow,
                                               const int32_t* __restrict__ rowptr, const int32_t* __restrict__ rowsplit,
@@ -862,38 +879,30 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             c->inc_elem = pgfd::dupload(pat.inc_elem.data(), c->n_inc);
             c->inc_slot = pgfd::dupload(pat.inc_slot.data(), c->n_inc);
             {
-                // packed records of the gather pass: one 16-byte load per
+                // packed records of the gather pass: one 8-byte load per
                 // incidence, one 16-byte load per element, one or two per node.
-                // Uncut incidence: (element, slot | material << 16, 0, 0).  Cut
-                // incidence: (first K_e row of the slot in the scratch, slot |
-                // 0x100 | M << 9 | material << 16, first column byte of the
-                // slot's rows, first P_e entry of the slot), so the gather reads
-                // the K_e row without a load of the element's scratch offsets.
-                // The offsets follow the scratch layout set up below (ke_ptr,
-                // ce_rel_ptr, pe_ptr).
-                std::vector<int64_t> kb(d->n_cut + 1, 0), rb(d->n_cut + 1, 0), pb(d->n_cut + 1, 0);
-                for (int64_t k = 0; k < d->n_cut; ++k) {
-                    const int m = ce_nt[k] * d->dim;
-                    kb[k + 1] = kb[k] + (int64_t)m * pgfk::cut_ke_stride(m);
-                    rb[k + 1] = rb[k] + (int64_t)ce_nt[k] * pgfk::cut_rel_stride(m);
-                    pb[k + 1] = pb[k] + m;
-                }
-                if (kb[d->n_cut] > INT32_MAX || rb[d->n_cut] > INT32_MAX)
-                    throw pgf::Error(PGF_ELIMIT, "cut-element scratch beyond 2^31 entries");
-                std::vector<int32_t> rec((size_t)std::max<int64_t>(c->n_inc, 1) * 4, 0);
+                // Uncut incidence: (element, slot | material << 16).  Cut
+                // incidence: (the slot's block in the scratch, slot | 0x100 |
+                // M << 9 | material << 16): the block holds the slot's K_e
+                // rows, their column bytes and P_e entries (cut_slot_doubles),
+                // so the gather reads them without a load of the element's
+                // offsets.  The offsets follow ke_ptr, set up below.
+                std::vector<int64_t> kb(d->n_cut + 1, 0);
+                for (int64_t k = 0; k < d->n_cut; ++k)
+                    kb[k + 1] = kb[k] + (int64_t)ce_nt[k] * pgfk::cut_slot_doubles(ce_nt[k] * d->dim, d->dim);
+                if (kb[d->n_cut] > INT32_MAX) throw pgf::Error(PGF_ELIMIT, "cut-element scratch beyond 2^31 entries");
+                std::vector<int32_t> rec((size_t)std::max<int64_t>(c->n_inc, 1) * 2, 0);
                 for (int64_t k = 0; k < c->n_inc; ++k) {
                     const int64_t e = pat.inc_elem[k];
                     const int32_t ci = mv.cut_index[e];
                     const int32_t sl = pat.inc_slot[k], mat = (int32_t)d->elem_mat[e] << 16;
                     if (ci >= 0) {
                         const int m = ce_nt[ci] * d->dim;
-                        rec[4 * k] = (int32_t)(kb[ci] + (int64_t)sl * d->dim * pgfk::cut_ke_stride(m));
-                        rec[4 * k + 1] = sl | 0x100 | (m << 9) | mat;
-                        rec[4 * k + 2] = (int32_t)(rb[ci] + (int64_t)sl * pgfk::cut_rel_stride(m));
-                        rec[4 * k + 3] = (int32_t)(pb[ci] + (int64_t)sl * d->dim);
+                        rec[2 * k] = (int32_t)(kb[ci] + (int64_t)sl * pgfk::cut_slot_doubles(m, d->dim));
+                        rec[2 * k + 1] = sl | 0x100 | (m << 9) | mat;
                     } else {
-                        rec[4 * k] = (int32_t)e;
-                        rec[4 * k + 1] = sl | mat;
+                        rec[2 * k] = (int32_t)e;
+                        rec[2 * k + 1] = sl | mat;
                     }
                 }
                 c->inc_rec = pgfd::dupload(rec.data(), (int64_t)rec.size());
@@ -934,13 +943,19 @@ PGF_API int pgf_create(const pgf_mesh_desc* d, pgf_ctx** out) {
             std::vector<int64_t> kp(d->n_cut + 1, 0), pp(d->n_cut + 1, 0);
             for (int64_t k = 0; k < d->n_cut; ++k) {
                 const int64_t m = (int64_t)ce_nt[k] * d->dim;
-                kp[k + 1] = kp[k] + m * pgfk::cut_ke_stride((int)m);
+                kp[k + 1] = kp[k] + (int64_t)ce_nt[k] * pgfk::cut_slot_doubles((int)m, d->dim);
                 pp[k + 1] = pp[k] + m;
             }
             c->n_ke = kp[d->n_cut];
             c->ke_ptr = pgfd::dupload(kp.data(), d->n_cut + 1);
             c->pe_ptr = pgfd::dupload(pp.data(), d->n_cut + 1);
             c->ke = pgfd::dalloc<double>(kp[d->n_cut]);
+            if (d->n_cut > 0) {
+                hipLaunchKernelGGL(pgfd::k_place_rel, dim3((unsigned)((d->n_cut + 255) / 256)), dim3(256), 0, c->stream,
+                                   d->n_cut, d->dim, c->ce_nt, c->ce_rel, c->ce_rel_ptr, c->ke_ptr, c->ke);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipStreamSynchronize(c->stream));
+            }
             c->pe = pgfd::dalloc<double>(pp[d->n_cut]);
             c->h_dof.assign(d->dof, d->dof + (size_t)d->dim * d->n_all_nodes);
             if (pat.shared_dofs) {

@@ -33,6 +33,15 @@ constexpr int GATHER_THREADS = 256;
 // bytes with a stride rounded up to 4 (read as 32-bit words)
 __host__ __device__ constexpr int cut_ke_stride(int m) { return (m + 1) & ~1; }
 __host__ __device__ constexpr int cut_rel_stride(int m) { return (m + 3) & ~3; }
+// The scratch holds, per slot of a cut element, its DIM rows of K_e (stride
+// cut_ke_stride), the rows' column bytes (cut_rel_stride bytes, placed once at
+// set-up) and the slot's DIM entries of P_e: what one incidence of the gather
+// reads sits together, addressed by one offset.  Sizes in doubles; a slot
+// block has an even size, so every block and row starts 16-byte aligned.
+__host__ __device__ constexpr int cut_rel_doubles(int m) { return (cut_rel_stride(m) + 7) / 8; }
+__host__ __device__ constexpr int cut_slot_doubles(int m, int dim) {
+    return (dim * cut_ke_stride(m) + cut_rel_doubles(m) + dim + 1) & ~1;
+}
 constexpr int SEG_MAX = 4096;
 // MI355X dispatches consecutive workgroups to its 8 XCDs in turn, each XCD
 // with its own L2.  Launching 8*per blocks and mapping block b to logical
@@ -97,12 +106,11 @@ struct AsmArgs {
     const int32_t* inc_elem;
     const uint8_t* inc_slot;
     const uint8_t* inc_rel;    // [n_inc][4 * dim] row-relative column positions (uncut elements)
-    // gather records: per incidence (element, slot | material << 16, 0, 0) or,
-    // for a cut element, (first K_e row of the slot in the scratch, slot |
-    // 0x100 | M << 9 | material << 16, first column byte of the slot's rows,
-    // first P_e entry of the slot); per element its 4 nodes; per node its
-    // coordinates padded to 2 or 4 doubles (one or two 16-byte loads)
-    const int4* inc_rec;       // [n_inc]
+    // gather records: per incidence (element, slot | material << 16) or, for a
+    // cut element, (the slot's block in the scratch, slot | 0x100 | M << 9 |
+    // material << 16); per element its 4 nodes; per node its coordinates
+    // padded to 2 or 4 doubles (one or two 16-byte loads)
+    const int2* inc_rec;       // [n_inc]
     const int4* el_conn4;      // [n_elem]
     const double* Xq;          // [n_all][2] (2D) or [n_all][4] (3D)
     double* ke;                // cut-element K_e scratch
